@@ -1,0 +1,264 @@
+"""ctypes binding of the CPU oracle (oracle/pppm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product path never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle_pppm.so")
+_lib = None
+
+KERNEL_EXPCUBIC, KERNEL_GAUSSIAN = 0, 1
+MG_VCYCLE, MG_REF_FMG = 0, 1
+PROLONG_CONSTANT, PROLONG_TRILINEAR = 0, 1
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("g", C.c_uint32), ("hf", C.c_float), ("originf", C.c_float * 3),
+        ("origin", C.c_double * 3), ("K", C.c_int), ("kernel", C.c_int),
+        ("core_a", C.c_double), ("mg_schedule", C.c_int), ("nu1", C.c_int),
+        ("nu2", C.c_int), ("mg_omega", C.c_double), ("mg_tol", C.c_double),
+        ("mg_max_cycles", C.c_int), ("mg_prolong", C.c_int),
+    ]
+
+
+class OrcDiag(C.Structure):
+    _fields_ = [
+        ("mg_cycles", C.c_int * 3), ("mg_relres", C.c_double * 3), ("strays", C.c_uint32),
+        ("t_hash", C.c_double), ("t_p2m", C.c_double), ("t_mg", C.c_double),
+        ("t_far", C.c_double), ("t_eval", C.c_double),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, OpenMP, no fast-math)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_cell_key.restype = C.c_uint32
+        _lib.orc_kernel_coeff.restype = C.c_double
+        _lib.orc_mg_norm2.restype = C.c_double
+        _lib.orc_mg_relres.restype = C.c_double
+        _lib.orc_mg_solve.restype = C.c_int
+        _lib.orc_mg_levels.restype = C.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ---- particle_hash ---------------------------------------------------------
+def hash_keys(pos4, g, hf, originf):
+    pos4 = _f32(pos4)
+    n = pos4.shape[0]
+    key = np.empty(n, np.uint32)
+    idx = np.empty(n, np.uint32)
+    of = (C.c_float * 3)(*[float(v) for v in originf])
+    lib().orc_hash_keys(_p(pos4), C.c_uint32(n), C.c_uint32(g), C.c_float(hf), of, _p(key), _p(idx))
+    return key, idx
+
+
+def sort_pairs(key, idx):
+    key = _u32(key).copy()
+    idx = _u32(idx).copy()
+    lib().orc_sort_pairs(_p(key), _p(idx), C.c_uint32(key.size))
+    return key, idx
+
+
+def cell_tables(key_sorted, ncells):
+    key_sorted = _u32(key_sorted)
+    start = np.empty(ncells, np.uint32)
+    end = np.empty(ncells, np.uint32)
+    lib().orc_cell_tables(_p(key_sorted), C.c_uint32(key_sorted.size), C.c_uint32(ncells), _p(start), _p(end))
+    return start, end
+
+
+# ---- P2M -------------------------------------------------------------------
+def p2m(start, end, pos4_sorted, mass_sorted, cell_h, g, origin):
+    grid = np.empty(g ** 3, np.float64)
+    o = (C.c_double * 3)(*[float(v) for v in origin])
+    pos4_sorted = _f32(pos4_sorted)
+    lib().orc_p2m(_p(_u32(start)), _p(_u32(end)), _p(pos4_sorted), _p(_f64(mass_sorted)),
+                  C.c_double(cell_h), _p(grid), C.c_uint32(g), C.c_uint32(pos4_sorted.shape[0]), o)
+    return grid
+
+
+# ---- multigrid --------------------------------------------------------------
+def mg_levels(nx, ny, nz):
+    dims = (C.c_int * 3 * 16)()
+    L = lib().orc_mg_levels(nx, ny, nz, dims, 16)
+    return [tuple(dims[i]) for i in range(L)]
+
+
+def mg_smooth(x, b, dims, iters, omega):
+    x = _f64(x).copy()
+    lib().orc_mg_smooth(_p(x), _p(_f64(b)), *[C.c_int(d) for d in dims], C.c_int(iters), C.c_double(omega))
+    return x
+
+
+def mg_residual(x, b, dims):
+    r = np.empty(int(np.prod(dims)), np.float64)
+    lib().orc_mg_residual(_p(_f64(x)), _p(_f64(b)), _p(r), *[C.c_int(d) for d in dims])
+    return r
+
+
+def mg_restrict(rf, cdims):
+    bc = np.empty(int(np.prod(cdims)), np.float64)
+    lib().orc_mg_restrict(_p(bc), _p(_f64(rf)), *[C.c_int(d) for d in cdims])
+    return bc
+
+
+def mg_prolong(xf, xc, fdims):
+    xf = _f64(xf).copy()
+    lib().orc_mg_prolong(_p(xf), _p(_f64(xc)), *[C.c_int(d) for d in fdims])
+    return xf
+
+
+def mg_coarse_exact(b, dims):
+    x = np.empty(int(np.prod(dims)), np.float64)
+    lib().orc_mg_coarse_exact(_p(x), _p(_f64(b)), *[C.c_int(d) for d in dims])
+    return x
+
+
+def mg_solve(b, dims, schedule=MG_VCYCLE, nu1=2, nu2=2, omega=1.0, tol=1e-10, max_cycles=50, x0=None,
+             prolong=PROLONG_TRILINEAR):
+    x = np.zeros(int(np.prod(dims)), np.float64) if x0 is None else _f64(x0).copy()
+    rel = C.c_double(0)
+    cyc = lib().orc_mg_solve(_p(x), _p(_f64(b)), *[C.c_int(d) for d in dims], C.c_int(schedule),
+                             C.c_int(nu1), C.c_int(nu2), C.c_double(omega), C.c_double(tol),
+                             C.c_int(max_cycles), C.c_int(prolong), C.byref(rel))
+    return x, cyc, rel.value
+
+
+def mg_relres(x, b, dims):
+    return lib().orc_mg_relres(_p(_f64(x)), _p(_f64(b)), *[C.c_int(d) for d in dims])
+
+
+# ---- curl / far field / M2P / near field ------------------------------------
+def curl(F, dims, h):
+    Fx, Fy, Fz = (_f64(f) for f in F)
+    n = int(np.prod(dims))
+    u, v, w = (np.empty(n, np.float64) for _ in range(3))
+    lib().orc_curl(_p(Fx), _p(Fy), _p(Fz), _p(u), _p(v), _p(w), *[C.c_int(d) for d in dims], C.c_double(h))
+    return u, v, w
+
+
+def far_dir_table(K, h):
+    W = 2 * K + 1
+    d = np.empty(W ** 3 * 3, np.float64)
+    lib().orc_far_dir_table(C.c_int(K), C.c_double(h), _p(d))
+    return d.reshape(W, W, W, 3)
+
+
+def far_field(wg, U, h, g, K):
+    wx, wy, wz = (_f64(a) for a in wg)
+    Ux, Uy, Uz = (_f64(a) for a in U)
+    fx, fy, fz = (np.empty(g ** 3, np.float64) for _ in range(3))
+    lib().orc_far_field(_p(wx), _p(wy), _p(wz), _p(Ux), _p(Uy), _p(Uz), _p(fx), _p(fy), _p(fz),
+                        C.c_double(h), C.c_int(g), C.c_int(K))
+    return fx, fy, fz
+
+
+def kernel_coeff(r, kernel, a):
+    return lib().orc_kernel_coeff(C.c_double(r), C.c_int(kernel), C.c_double(a))
+
+
+# ---- whole path -------------------------------------------------------------
+def make_params(g, h, origin, K=2, kernel=KERNEL_EXPCUBIC, core_a=None, schedule=MG_VCYCLE,
+                nu1=2, nu2=2, omega=1.0, tol=1e-12, max_cycles=60, prolong=None):
+    """Parameters as the reference derives them: the hash stores (float)h and
+    (float3)origin; P2M/M2P/near field use (double)(float)h and the double origin."""
+    p = OrcParams()
+    p.g = g
+    p.hf = float(np.float32(h))
+    for i in range(3):
+        p.originf[i] = float(np.float32(origin[i]))
+        p.origin[i] = float(origin[i])
+    p.K = K
+    p.kernel = kernel
+    p.core_a = 1.5 * float(np.float32(h)) if core_a is None else float(core_a)
+    p.mg_schedule = schedule
+    p.nu1, p.nu2 = nu1, nu2
+    p.mg_omega = omega
+    p.mg_tol = tol
+    p.mg_max_cycles = max_cycles
+    if prolong is None:
+        prolong = PROLONG_CONSTANT if schedule == MG_REF_FMG else PROLONG_TRILINEAR
+    p.mg_prolong = prolong
+    return p
+
+
+def pppm_eval(params, vort_pos4, omega, eval_pos4=None, other_end4=None):
+    """FP64 PPPM velocity at eval points (default: at the vortices themselves).
+    omega: (3, N) float64.  Returns (u (3, M) float64, diag)."""
+    vort_pos4 = _f32(vort_pos4)
+    n = vort_pos4.shape[0]
+    if eval_pos4 is None:
+        eval_pos4 = vort_pos4
+    eval_pos4 = _f32(eval_pos4)
+    m = eval_pos4.shape[0]
+    wx, wy, wz = (_f64(omega[i]) for i in range(3))
+    ux, uy, uz = (np.empty(m, np.float64) for _ in range(3))
+    diag = OrcDiag()
+    oth = None if other_end4 is None else _f32(other_end4)
+    lib().orc_pppm_eval(C.byref(params), C.c_uint32(n), _p(vort_pos4), _p(wx), _p(wy), _p(wz),
+                        C.c_uint32(m), _p(eval_pos4), None if oth is None else _p(oth),
+                        _p(ux), _p(uy), _p(uz), C.byref(diag))
+    return np.stack([ux, uy, uz]), diag
+
+
+def direct_sum(vort_pos4, omega, eval_pos4=None, kernel=KERNEL_EXPCUBIC, core_a=0.08):
+    vort_pos4 = _f32(vort_pos4)
+    if eval_pos4 is None:
+        eval_pos4 = vort_pos4
+    eval_pos4 = _f32(eval_pos4)
+    m = eval_pos4.shape[0]
+    wx, wy, wz = (_f64(omega[i]) for i in range(3))
+    ux, uy, uz = (np.empty(m, np.float64) for _ in range(3))
+    lib().orc_direct_sum(C.c_uint32(vort_pos4.shape[0]), _p(vort_pos4), _p(wx), _p(wy), _p(wz),
+                         C.c_uint32(m), _p(eval_pos4), C.c_int(kernel), C.c_double(core_a),
+                         _p(ux), _p(uy), _p(uz))
+    return np.stack([ux, uy, uz])
+
+
+def reference_domain(pos4, g):
+    pos4 = _f32(pos4)
+    o = (C.c_double * 3)()
+    L = C.c_double(0)
+    lib().orc_reference_domain(_p(pos4), C.c_uint32(pos4.shape[0]), C.c_uint32(g), o, C.byref(L))
+    return [o[0], o[1], o[2]], L.value
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def mg_prolong_tri(xf, xc, fdims):
+    xf = _f64(xf).copy()
+    lib().orc_mg_prolong_tri(_p(xf), _p(_f64(xc)), *[C.c_int(d) for d in fdims])
+    return xf

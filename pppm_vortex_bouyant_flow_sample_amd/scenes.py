@@ -1,0 +1,90 @@
+"""Seeded synthetic scenes for the PPPM vortex-velocity workloads.
+
+Positions are float32 (x, y, z, w=0) rows, strengths float64 (3, N).  Every
+scene is a pure function of its seed (numpy PCG64), so the host-side inputs of
+a GPU run and of its CPU check are bit-identical.
+
+  uniform : positions U[0,1)^3, strengths N(0,1)^3 (65,536 particles, 32^3 grid)
+  ring    : vortex ring of major radius 0.25 about z through (1/2,1/2,1/2);
+            cross-section offset from a 2-D Gaussian (sigma 0.02) resampled
+            beyond 0.05; strength tangent to the ring, magnitude 1/N
+            (2,097,152 particles, 128^3 grid)
+  plume   : x,y ~ N(0.5, 0.08), z ~ Exp(rate 3), all clipped to [0,1);
+            strengths N(0,1)^3 * exp(-3 z); smoke density U(0,1)
+            (8,388,608 particles, 256^3 grid)
+"""
+import numpy as np
+
+SEED_BASE = 20260000
+ONE_MINUS = np.float32(np.nextafter(np.float32(1.0), np.float32(0.0)))
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _pack(x, y, z):
+    n = x.shape[0]
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = x
+    pos[:, 1] = y
+    pos[:, 2] = z
+    return pos
+
+
+def _clip01(v):
+    v = np.asarray(v, np.float32)
+    return np.minimum(np.maximum(v, np.float32(0.0)), ONE_MINUS)
+
+
+def uniform(n, seed=SEED_BASE + 0):
+    r = _rng(seed)
+    p = r.random((3, n), dtype=np.float64)
+    pos = _pack(_clip01(p[0]), _clip01(p[1]), _clip01(p[2]))
+    w = r.standard_normal((3, n))
+    return pos, w
+
+
+def ring(n, seed=SEED_BASE + 1, R=0.25, sigma=0.02, rmax=0.05):
+    r = _rng(seed)
+    theta = r.random(n) * (2.0 * np.pi)
+    off = r.normal(0.0, sigma, (2, n))
+    bad = np.hypot(off[0], off[1]) > rmax
+    while bad.any():
+        k = int(bad.sum())
+        off[:, bad] = r.normal(0.0, sigma, (2, k))
+        bad = np.hypot(off[0], off[1]) > rmax
+    rad = R + off[0]
+    x = 0.5 + rad * np.cos(theta)
+    y = 0.5 + rad * np.sin(theta)
+    z = 0.5 + off[1]
+    pos = _pack(_clip01(x), _clip01(y), _clip01(z))
+    w = np.stack([-np.sin(theta), np.cos(theta), np.zeros(n)]) / n
+    return pos, w
+
+
+def plume(n, seed=SEED_BASE + 2, sigma=0.08, rate=3.0):
+    r = _rng(seed)
+    x = r.normal(0.5, sigma, n)
+    y = r.normal(0.5, sigma, n)
+    z = r.exponential(1.0 / rate, n)
+    pos = _pack(_clip01(x), _clip01(y), _clip01(z))
+    zz = pos[:, 2].astype(np.float64)
+    w = r.standard_normal((3, n)) * np.exp(-3.0 * zz)[None, :]
+    rho = r.random(n)
+    return pos, w, rho
+
+
+WORKLOADS = {
+    # name: (generator, N, g)
+    "uniform-64k-32": (uniform, 65536, 32),
+    "ring-2M-128": (ring, 2097152, 128),
+    "plume-8M-256": (plume, 8388608, 256),
+}
+
+
+def make(name, n=None, seed=None):
+    gen, n0, g = WORKLOADS[name]
+    n = n0 if n is None else n
+    out = gen(n) if seed is None else gen(n, seed=seed)
+    return out[0], out[1], g
