@@ -1,0 +1,192 @@
+/*
+ * pppm.h -- C ABI of the gfx950 PPPM vortex-velocity library (libpppm_hip.so).
+ *
+ * Plain pointers and sizes only.  All array arguments are DEVICE pointers
+ * unless stated otherwise; all work is enqueued on the context's HIP stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   pppm_set_vortices      BiotSavartSolver::setVortParameter
+ *                          (hybridFastSummation/hybridFastSummation/BiotSavartSolver.h:33,
+ *                           BiotSavartSolverImp.cpp:318-377)
+ *   pppm_compute_far_field BiotSavartSolver::computeFarFieldBuffer
+ *                          (BiotSavartSolver.h:36, BiotSavartSolverImp.cpp:755-816)
+ *   pppm_evaluate          BiotSavartSolver::setEvalParameter + evaluateVelocity + getU
+ *                          (BiotSavartSolver.h:23-24,32; BiotSavartSolverImp.cpp:266-316,410-516)
+ *   pppm_hash_*            ParticleManager::doSpatialHash / reorderData / deorderData
+ *                          (particle_hash/particle_hash/SpaceHashClass.h:42-69,
+ *                           SpaceHashClassImp.cpp:18-34)
+ *   pppm_mg_*              gf::MultiGridSolver3DPeriod(_double)::m_InitialSystem /
+ *                          m_FullMultiGrid / m_Vcycle (multigrid/multigrid/MultiGrid.h:128-209,
+ *                          MultiGrid.cpp:128-250)
+ * The reference's own extern "C" names (calcHash, FindCellStart, sortParticles,
+ * reorder, deorder, setParameters, mg_*, AllocateMemory, ...) are declared in
+ * pppm_ref.h and implemented on top of these.
+ */
+#ifndef PPPM_H
+#define PPPM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pppm_status {
+    PPPM_OK = 0,
+    PPPM_ERR_INVALID = 1,       /* bad argument or unsupported size            */
+    PPPM_ERR_HIP = 2,           /* HIP runtime error (pppm_last_error() says which) */
+    PPPM_ERR_NOMEM = 3,         /* device allocation failed                    */
+    PPPM_ERR_BUDGET = 4,        /* request larger than the HBM budget (serving) */
+    PPPM_ERR_NOT_CONVERGED = 5, /* multigrid stopped above its tolerance       */
+    PPPM_ERR_STATE = 6,         /* call out of order (e.g. evaluate before far field) */
+    PPPM_ERR_NO_DEVICE = 7      /* no usable GPU                               */
+} pppm_status;
+
+/* element types; the values equal the reference's "precision" codes
+ * (particle_hash_kernel.cu:143-152: 1 = float, 2 = double) */
+enum { PPPM_F32 = 1, PPPM_F64 = 2 };
+enum { PPPM_KERNEL_EXPCUBIC = 0, PPPM_KERNEL_GAUSSIAN = 1 };
+enum { PPPM_MG_VCYCLE = 0, PPPM_MG_REF_FMG = 1 };
+enum { PPPM_PREC_FP32 = 0, PPPM_PREC_MIXED = 1 };
+
+typedef struct pppm_params {
+    uint32_t g;             /* grid cells per axis (cubic grid and hash), 8..1024      */
+    double h;               /* cell size; the hash keeps (float)h, the path (double)(float)h */
+    double origin[3];       /* domain origin; the hash keeps (float)origin              */
+    int32_t K;              /* near-field range in cells, 0..4                          */
+    int32_t kernel;         /* PPPM_KERNEL_*                                           */
+    double core_a;          /* smoothing core radius (absolute units)                 */
+    int32_t mg_schedule;    /* PPPM_MG_VCYCLE (to tolerance) or PPPM_MG_REF_FMG          */
+    int32_t nu1, nu2;       /* red+black smoothing iterations before / after          */
+    double mg_omega;        /* SOR factor (1 = Gauss-Seidel; the reference uses 1.43f)  */
+    double mg_tol;          /* relative residual target (VCYCLE / mixed)                */
+    int32_t mg_max_cycles;  /* cap on V-cycles (plain) or outer corrections (mixed)    */
+    int32_t precision;      /* PPPM_PREC_FP32 or PPPM_PREC_MIXED                        */
+} pppm_params;
+
+typedef struct pppm_stats {
+    uint32_t n, m;            /* vortices, evaluation points of the last calls        */
+    int32_t mg_cycles;        /* V-cycles run (mixed: total inner V-cycles)           */
+    int32_t mg_outer;         /* FP64 defect-correction steps (mixed), else 0         */
+    double mg_relres[3];      /* final relative residual per component (FP64 norms) */
+    uint32_t strays;          /* vortices whose deposit cell differs from hash cell   */
+    int32_t mg_converged;     /* 1 if the tolerance was met (REF_FMG: always 1)       */
+} pppm_stats;
+
+typedef struct pppm_ctx pppm_ctx;
+
+/* Defaults for the benchmark configurations: domain [0,1)^3, h = 1/g, K = 2,
+ * expcubic core a = 1.5 h, V(2,2) Gauss-Seidel cycles to relative residual 1e-6. */
+void pppm_params_config_preset(pppm_params *p, uint32_t g);
+/* The reference's own settings (BiotSavartSolverImp.cpp, VortexSimulationSystemImp.cpp:890-905):
+ * a = 0.08, one FMG pass with V(4,4) SOR 1.43f; origin/h from pppm_reference_domain. */
+void pppm_params_reference_preset(pppm_params *p, uint32_t g, int32_t K);
+/* BiotSavartSolverImp.cpp:378-409 (setDomain): host positions (float4 rows) ->
+ * origin = bbox centre - L/2, L = 3 * largest extent; writes h = L / g. */
+pppm_status pppm_reference_domain(const float *host_pos4, uint32_t n, uint32_t g, double origin_out[3],
+                                  double *h_out);
+
+pppm_status pppm_create(const pppm_params *p, pppm_ctx **out);
+void pppm_destroy(pppm_ctx *ctx);
+pppm_status pppm_set_stream(pppm_ctx *ctx, void *hip_stream);
+pppm_status pppm_get_stream(const pppm_ctx *ctx, void **hip_stream);
+pppm_status pppm_get_params(const pppm_ctx *ctx, pppm_params *out);
+
+/* Vortices: pos4 = float4 rows (x, y, z, unused); strength component c of
+ * vortex i at w_c[i * w_stride] with element type w_dtype (PPPM_F32/PPPM_F64).
+ * Hashes, sorts and reorders them (device-resident afterwards). */
+pppm_status pppm_set_vortices(pppm_ctx *ctx, uint32_t n, const float *pos4, const void *wx, const void *wy,
+                              const void *wz, uint32_t w_stride, int32_t w_dtype);
+/* P2M, three Poisson solves, curl and far-field buffer. */
+pppm_status pppm_compute_far_field(pppm_ctx *ctx);
+/* Velocity at m points (float4 rows), written in the caller's order to
+ * u_c[i * u_stride] with element type u_dtype.  other_end4 (may be NULL) is the
+ * segment mode of the reference: vortices within 1e-12 of the other end are skipped. */
+pppm_status pppm_evaluate(pppm_ctx *ctx, uint32_t m, const float *eval_pos4, const float *other_end4, void *ux,
+                          void *uy, void *uz, uint32_t u_stride, int32_t u_dtype);
+/* Velocity at the vortices themselves (reuses the vortex hash). */
+pppm_status pppm_evaluate_self(pppm_ctx *ctx, void *ux, void *uy, void *uz, uint32_t u_stride, int32_t u_dtype);
+/* Whole path on the vortices: set_vortices + compute_far_field + evaluate_self. */
+pppm_status pppm_solve_self(pppm_ctx *ctx, uint32_t n, const float *pos4, const void *wx, const void *wy,
+                            const void *wz, uint32_t w_stride, int32_t w_dtype, void *ux, void *uy, void *uz,
+                            uint32_t u_stride, int32_t u_dtype);
+pppm_status pppm_get_stats(const pppm_ctx *ctx, pppm_stats *out);
+
+/* Sorted device views of the last set_vortices (BiotSavartSolver::getSortPos /
+ * getVortStartTable / getVortEndTable / getSortVort, BiotSavartSolver.h:25-29).
+ * Positions are float4 (x, y, z, w) in the caller's coordinates. */
+pppm_status pppm_get_sorted_vortices(pppm_ctx *ctx, const float **pos4, const uint32_t **sorted_index,
+                                     const uint32_t **cell_start, const uint32_t **cell_end);
+/* Grid fields after compute_far_field: which = 0 vorticity (h^-3 scaled),
+ * 1 stream function, 2 far-field velocity; component c; float32 [g^3]
+ * (mixed precision: the stream function is float64). */
+pppm_status pppm_get_grid(pppm_ctx *ctx, int32_t which, int32_t c, const void **ptr, int32_t *dtype);
+
+/* ---- particle hash (ParticleManager, SpaceHashClass.h:10-84) -------------- */
+typedef struct pppm_hash pppm_hash;
+pppm_status pppm_hash_create(uint32_t capacity, uint32_t g, float h, const float origin[3], pppm_hash **out);
+void pppm_hash_destroy(pppm_hash *hs);
+pppm_status pppm_hash_set_grid(pppm_hash *hs, uint32_t g, float h, const float origin[3]);
+/* keys, stable sort by (key, index), start/end tables (empty cells 0xFFFFFFFF) */
+pppm_status pppm_hash_build(pppm_hash *hs, const float *pos4, uint32_t n, void *hip_stream);
+pppm_status pppm_hash_tables(pppm_hash *hs, const uint32_t **key_sorted, const uint32_t **index_sorted,
+                             const uint32_t **cell_start, const uint32_t **cell_end);
+/* out[i*stride+c] = in[index[i]*stride+c] (reorder) and its inverse (deorder) */
+pppm_status pppm_hash_reorder(pppm_hash *hs, uint32_t n, const void *in, void *out, uint32_t stride,
+                              int32_t dtype, void *hip_stream);
+pppm_status pppm_hash_deorder(pppm_hash *hs, uint32_t n, const void *in, void *out, uint32_t stride,
+                              int32_t dtype, void *hip_stream);
+
+/* ---- primitives on caller buffers ----------------------------------------- */
+/* keys[i] = cell key of pos4[i], idx[i] = i (particle_hash_kernel.cu:32-70) */
+pppm_status pppm_calc_keys(const float *pos4, uint32_t n, uint32_t g, float h, const float origin[3],
+                           uint32_t *keys, uint32_t *idx, void *hip_stream);
+/* stable ascending sort of (keys, vals) pairs by key; scratch sized by
+ * pppm_sort_scratch_bytes; key_bits = significant low bits of the keys */
+size_t pppm_sort_scratch_bytes(uint32_t n);
+pppm_status pppm_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t n, uint32_t key_bits, void *scratch,
+                            void *hip_stream);
+pppm_status pppm_cell_tables(const uint32_t *keys_sorted, uint32_t n, uint32_t ncells, uint32_t *start,
+                             uint32_t *end, void *hip_stream);
+pppm_status pppm_gather(const uint32_t *idx, const void *in, void *out, uint32_t stride, int32_t dtype,
+                        uint32_t n, void *hip_stream);
+pppm_status pppm_scatter(const uint32_t *idx, const void *in, void *out, uint32_t stride, int32_t dtype,
+                         uint32_t n, void *hip_stream);
+
+/* ---- multigrid on caller grids (MultiGrid.h:29-209) ------------------------- */
+typedef struct pppm_mg pppm_mg;
+/* dtype PPPM_F32 or PPPM_F64; nx,ny,nz arbitrary >= 1 */
+pppm_status pppm_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, pppm_mg **out);
+void pppm_mg_destroy(pppm_mg *mg);
+pppm_status pppm_mg_levels(const pppm_mg *mg, int32_t *levels, int32_t dims_out[][3], int32_t max_levels);
+/* Solve sum_nb(x) - 6 x = b (zero ghosts) in place on x.  schedule
+ * PPPM_MG_REF_FMG = one full-multigrid pass + one V-cycle (the reference);
+ * PPPM_MG_VCYCLE = V(nu1,nu2) cycles until ||b - A x|| / ||b|| <= tol.
+ * cycles_out / relres_out may be NULL. */
+pppm_status pppm_mg_solve(pppm_mg *mg, void *x, const void *b, int32_t schedule, int32_t nu1, int32_t nu2,
+                          double omega, double tol, int32_t max_cycles, int32_t *cycles_out,
+                          double *relres_out, void *hip_stream);
+
+/* ---- device memory (MemoryOperation.cuh:13-29 semantics) -------------------- */
+pppm_status pppm_device_count(int32_t *n);
+pppm_status pppm_malloc(void **ptr, size_t bytes);
+pppm_status pppm_free(void *ptr);
+pppm_status pppm_memcpy_htod(void *dst, const void *src, size_t bytes);
+pppm_status pppm_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+pppm_status pppm_memcpy_dtod(void *dst, const void *src, size_t bytes);
+pppm_status pppm_memset(void *dst, int value, size_t bytes);
+pppm_status pppm_synchronize(void);
+const char *pppm_last_error(void);
+
+/* ---- timing on the library's streams --------------------------------------- */
+typedef struct pppm_timer pppm_timer;
+pppm_status pppm_timer_create(pppm_timer **out);
+void pppm_timer_destroy(pppm_timer *t);
+pppm_status pppm_timer_start(pppm_timer *t, void *hip_stream);
+pppm_status pppm_timer_stop(pppm_timer *t, void *hip_stream);
+pppm_status pppm_timer_elapsed_ms(pppm_timer *t, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

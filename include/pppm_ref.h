@@ -1,0 +1,86 @@
+/*
+ * pppm_ref.h -- the reference's own extern "C" entry points, implemented by
+ * libpppm_hip.so on gfx950 (same names, argument meaning and order; every
+ * call is enqueued on the legacy default stream like the reference's).
+ *
+ * Replaced interfaces (paths relative to the reference repository):
+ *   particle_hash/particle_hash/SpaceHashHeader.cuh:12-47   setParameters, calcHash,
+ *       FindCellStart, reorder, deorder, sortParticles
+ *   particle_hash/particle_hash/hash_parameters.h:9-19     HashParams
+ *   multigrid/multigrid/MultiGridcu.h:16-95                 mg_* (float and double "2")
+ *   multigrid/multigrid/MemoryOperation.cuh:13-29           InitCuda, AllocateMemory, ...
+ *
+ * Behaviour differences, all deliberate:
+ *   - sortParticles is a stable sort (order by (key, original index));
+ *   - FindCellStart also sets cellEnd = 0xFFFFFFFF for empty cells;
+ *   - mg_exact_periodic3D(2) solves the coarsest level exactly (the reference
+ *     runs 100 racy relaxations); mg_InitIdxBuffer is therefore not needed and
+ *     only records the coarsest dimensions;
+ *   - errors set pppm_last_error() instead of exiting the process.
+ * InitCudaGL (GL interop) has no counterpart: the path has no OpenGL.
+ */
+#ifndef PPPM_REF_H
+#define PPPM_REF_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hash_parameters.h:9-19 (uint3 / float3 members as arrays, same layout) */
+typedef struct HashParams {
+    uint32_t gridSize[3];
+    uint32_t numCells;
+    float worldOrigin[3];
+    float cellSize[3];
+    uint32_t maxParticlesPerCell;
+} HashParams;
+
+/* SpaceHashHeader.cuh:14 */
+void setParameters(HashParams *hostParams);
+/* SpaceHashHeader.cuh:16-19: key = cell of pos (float4 rows), index = i */
+void calcHash(uint32_t *gridParticleHash, uint32_t *gridParticleIndex, float *pos, int numParticles);
+/* SpaceHashHeader.cuh:21-27 */
+void FindCellStart(uint32_t *cellStart, uint32_t *cellEnd, uint32_t *gridParticleHash, uint32_t *gridParticleIndex,
+                   uint32_t numParticles, uint32_t numCells);
+/* SpaceHashHeader.cuh:29-35: out[i*stride+c] = data[index[i]*stride+c]; precision 1 float, 2 double */
+void reorder(uint32_t *indexTable, void *data, uint32_t stride, void *out_data, uint32_t precision, uint32_t num_data);
+/* SpaceHashHeader.cuh:37-43: out[index[i]*stride+c] = data[i*stride+c] */
+void deorder(uint32_t *indexTable, void *data, uint32_t stride, void *out_data, uint32_t precision, uint32_t num_data);
+/* SpaceHashHeader.cuh:45 */
+void sortParticles(uint32_t *dGridParticleHash, uint32_t *dGridParticleIndex, uint32_t numParticles);
+
+/* MultiGridcu.h:18-56 (float) */
+void mg_InitIdxBuffer(int dimx, int dimy, int dimz);
+void mg_exact_periodic3D(float *x, float *b, float *res, int dimx, int dimy, int dimz);
+void mg_zerofy(void *data, size_t size);
+void mg_RBGS_periodic3D(float *x, float *b, float *res, int t, int dimx, int dimy, int dimz);
+void mg_ComputeResidualPeriodic3D(float *x, float *b, float *r, int dimx, int dimy, int dimz);
+void mg_RestrictionPeriodic3D(float *next_level, float *curr_level, int next_dimx, int next_dimy, int next_dimz);
+void mg_ProlongationPeriodic3D(float *next_level, float *coaser_level, int next_dimx, int next_dimy, int next_dimz);
+/* MultiGridcu.h:61-92 (double) */
+void mg_exact_periodic3D2(double *x, double *b, double *res, int dimx, int dimy, int dimz);
+void mg_RBGS_periodic3D2(double *x, double *b, double *res, int t, int dimx, int dimy, int dimz);
+void mg_ComputeResidualPeriodic3D2(double *x, double *b, double *r, int dimx, int dimy, int dimz);
+void mg_RestrictionPeriodic3D2(double *next_level, double *curr_level, int next_dimx, int next_dimy, int next_dimz);
+void mg_ProlongationPeriodic3D2(double *next_level, double *coaser_level, int next_dimx, int next_dimy,
+                                int next_dimz);
+
+/* MemoryOperation.cuh:15-26 */
+void InitCuda(int argc, char *argv[]);
+void AllocateMemoryPitch(size_t size, size_t *pitch, void **devPtr);
+void AllocateMemory(size_t size, void **devPtr);
+void FreeMemory(void *devPtr);
+void ThreadSync(void);
+void MemcpyHstToDevPitch(void *devPtr, size_t dPitch, const void *hstPtr, size_t hPitch, size_t size);
+void MemcpyDevToHstPitch(const void *devPtr, size_t dPitch, void *hstPtr, size_t hPitch, size_t size);
+void MemcpyDevToDevPitch(void *dstPtr, size_t dPitch, const void *srcPtr, size_t sPitch, size_t size);
+void MemcpyHstToDev(void *devPtr, const void *hstPtr, size_t size);
+void MemcpyDevToHst(const void *devPtr, void *hstPtr, size_t size);
+void MemcpyDevToDev(void *dstPtr, const void *srcPtr, size_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -146,7 +146,9 @@ def mg_coarse_exact(b, dims):
 
 
 def mg_solve(b, dims, schedule=MG_VCYCLE, nu1=2, nu2=2, omega=1.0, tol=1e-10, max_cycles=50, x0=None,
-             prolong=PROLONG_TRILINEAR):
+             prolong=None):
+    if prolong is None:
+        prolong = PROLONG_CONSTANT if schedule == MG_REF_FMG else PROLONG_TRILINEAR
     x = np.zeros(int(np.prod(dims)), np.float64) if x0 is None else _f64(x0).copy()
     rel = C.c_double(0)
     cyc = lib().orc_mg_solve(_p(x), _p(_f64(b)), *[C.c_int(d) for d in dims], C.c_int(schedule),

@@ -1,0 +1,655 @@
+// C ABI: context, pipeline orchestration, hash / multigrid objects, memory.
+// Pipeline = BiotSavartSolver::computeFarFieldBuffer + evaluateVelocity
+// (hybridFastSummation/hybridFastSummation/BiotSavartSolverImp.cpp:410-816)
+// with the device-resident layout of this library: sorted float4 vortex
+// coordinates (origin-shifted, deposit cell in .w) and strengths, component-
+// major [3][g^3] grids, cell offsets [g^3 + 1].
+#include "mg.hpp"
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace pppm;
+
+static thread_local std::string g_last_error;
+
+extern "C" void pppm_set_last_error(const char* msg, const char* file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s (%s:%d)", msg ? msg : "?", file ? file : "?", line);
+    g_last_error = buf;
+}
+
+extern "C" const char* pppm_last_error(void) { return g_last_error.c_str(); }
+
+#define FAIL(code, msg)                                  \
+    do {                                                 \
+        pppm_set_last_error((msg), __FILE__, __LINE__);  \
+        return (code);                                   \
+    } while (0)
+#define TRY(expr)                            \
+    do {                                     \
+        pppm_status _s = (expr);             \
+        if (_s != PPPM_OK) return _s;        \
+    } while (0)
+
+namespace {
+
+template <typename T>
+pppm_status dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        FAIL(PPPM_ERR_NOMEM, "device allocation failed");
+    }
+    return PPPM_OK;
+}
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Sorted-particle buffers: keys / values ping-pong + radix scratch.
+struct SortBufs {
+    uint32_t cap = 0;
+    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    void* scratch = nullptr;
+    uint32_t *ks = nullptr, *vs = nullptr;  // result views
+    pppm_status ensure(uint32_t n) {
+        if (n <= cap && k0) return PPPM_OK;
+        release();
+        uint32_t c = n < 1024 ? 1024 : n;
+        TRY(dev_alloc(&k0, c));
+        TRY(dev_alloc(&v0, c));
+        TRY(dev_alloc(&k1, c));
+        TRY(dev_alloc(&v1, c));
+        uint8_t* s;
+        TRY(dev_alloc(&s, sort_scratch_bytes(c)));
+        scratch = s;
+        cap = c;
+        return PPPM_OK;
+    }
+    void release() {
+        dev_free(k0);
+        dev_free(v0);
+        dev_free(k1);
+        dev_free(v1);
+        uint8_t* s = (uint8_t*)scratch;
+        dev_free(s);
+        scratch = nullptr;
+        cap = 0;
+        ks = vs = nullptr;
+    }
+};
+
+pppm_status validate_params(const pppm_params* p) {
+    if (!p) FAIL(PPPM_ERR_INVALID, "null params");
+    if (p->g < 2 || p->g > 1024) FAIL(PPPM_ERR_INVALID, "g must be in [2, 1024]");
+    if (!(p->h > 0) || !std::isfinite(p->h)) FAIL(PPPM_ERR_INVALID, "h must be positive");
+    if (p->K < 0 || p->K > 4) FAIL(PPPM_ERR_INVALID, "K must be in [0, 4]");
+    if (p->kernel != PPPM_KERNEL_EXPCUBIC && p->kernel != PPPM_KERNEL_GAUSSIAN) FAIL(PPPM_ERR_INVALID, "kernel");
+    if (!(p->core_a > 0)) FAIL(PPPM_ERR_INVALID, "core_a must be positive");
+    if (p->mg_schedule != PPPM_MG_VCYCLE && p->mg_schedule != PPPM_MG_REF_FMG) FAIL(PPPM_ERR_INVALID, "schedule");
+    if (p->nu1 < 0 || p->nu2 < 0 || p->nu1 + p->nu2 == 0) FAIL(PPPM_ERR_INVALID, "smoothing counts");
+    if (p->mg_max_cycles < 1) FAIL(PPPM_ERR_INVALID, "mg_max_cycles");
+    if (p->precision != PPPM_PREC_FP32 && p->precision != PPPM_PREC_MIXED) FAIL(PPPM_ERR_INVALID, "precision");
+    if (p->precision == PPPM_PREC_MIXED && p->mg_schedule != PPPM_MG_VCYCLE)
+        FAIL(PPPM_ERR_INVALID, "mixed precision needs the V-cycle schedule");
+    return PPPM_OK;
+}
+
+PathGeom make_geom(const pppm_params& p) {
+    PathGeom pg;
+    pg.hash.g = p.g;
+    pg.hash.hf = (float)p.h;
+    for (int i = 0; i < 3; i++) {
+        pg.hash.of[i] = (float)p.origin[i];
+        pg.origin[i] = p.origin[i];
+    }
+    pg.h = (double)pg.hash.hf;  // the path uses the hash's float cell size
+    pg.invh = 1.0 / pg.h;
+    return pg;
+}
+
+// dir table of the far-field correction (GPU_kernel.cu:2646-2677), FP64 on the
+// host, stored as float3 rows in ((dz+K)(2K+1)+(dy+K))(2K+1)+(dx+K) order.
+std::vector<float> far_dir_table(int K, double h) {
+    const int W = 2 * K + 1;
+    std::vector<float> out((size_t)W * W * W * 3, 0.f);
+    auto R = [](double ax, double ay, double az, double bx, double by, double bz) {
+        double r2 = (ax - bx) * (ax - bx) + (ay - by) * (ay - by) + (az - bz) * (az - bz);
+        return r2 > 1e-12 ? kRefFourPi * std::sqrt(r2) : 1e24;
+    };
+    for (int kk = -K; kk <= K; kk++)
+        for (int jj = -K; jj <= K; jj++)
+            for (int ii = -K; ii <= K; ii++) {
+                if (ii == 0 && jj == 0 && kk == 0) continue;
+                double px = (ii + 0.5) * h, py = (jj + 0.5) * h, pz = (kk + 0.5) * h, X = 0.5 * h;
+                double dx = 0.5 * ((1.0 / R(X + h, X, X, px, py, pz)) - (1.0 / R(X - h, X, X, px, py, pz))) / h;
+                double dy = 0.5 * ((1.0 / R(X, X + h, X, px, py, pz)) - (1.0 / R(X, X - h, X, px, py, pz))) / h;
+                double dz = 0.5 * ((1.0 / R(X, X, X + h, px, py, pz)) - (1.0 / R(X, X, X - h, px, py, pz))) / h;
+                size_t o = 3 * (size_t)(((kk + K) * W + (jj + K)) * W + (ii + K));
+                out[o] = (float)dx;
+                out[o + 1] = (float)dy;
+                out[o + 2] = (float)dz;
+            }
+    return out;
+}
+
+}  // namespace
+
+namespace pppm {
+pppm_status launch_curl_far_impl(const void* psi, int psi_dtype, const float* wgrid, float* far, const float* dir,
+                                 uint32_t g, int K, double h, hipStream_t st);
+}
+
+struct pppm_ctx {
+    pppm_params p{};
+    PathGeom pg{};
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    uint32_t g = 0;
+    size_t G = 0;
+    // vortices
+    uint32_t n = 0, capN = 0;
+    SortBufs vs;
+    float4 *spos = nullptr, *somg = nullptr;
+    uint32_t *cstart = nullptr, *cend = nullptr, *coff = nullptr, *scan_scr = nullptr;
+    uint32_t* d_strays = nullptr;
+    uint32_t* h_strays = nullptr;
+    bool have_vort = false;
+    // evaluation points
+    uint32_t capM = 0;
+    SortBufs es;
+    float4 *epos = nullptr, *eoth = nullptr;
+    // grids
+    float *wg = nullptr, *far = nullptr, *psi32 = nullptr, *dir = nullptr;
+    double *psi64[2] = {nullptr, nullptr}, *b64 = nullptr;
+    float *r32 = nullptr, *e32 = nullptr;
+    int psi_cur = 0;
+    MGHier<float> mg;
+    bool have_far = false;
+    pppm_stats stats{};
+
+    pppm_status init(const pppm_params& prm) {
+        p = prm;
+        pg = make_geom(p);
+        g = p.g;
+        G = (size_t)g * g * g;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            FAIL(PPPM_ERR_HIP, "stream creation failed");
+        own_stream = true;
+        TRY(dev_alloc(&cstart, G));
+        TRY(dev_alloc(&cend, G));
+        TRY(dev_alloc(&coff, G + 1));
+        TRY(dev_alloc(&scan_scr, scan_scratch_words(G) + 16));
+        TRY(dev_alloc(&d_strays, 4));
+        if (hipHostMalloc((void**)&h_strays, 16) != hipSuccess) FAIL(PPPM_ERR_NOMEM, "pinned alloc failed");
+        TRY(dev_alloc(&wg, 3 * G));
+        TRY(dev_alloc(&far, 3 * G));
+        if (p.precision == PPPM_PREC_MIXED) {
+            TRY(dev_alloc(&psi64[0], 3 * G));
+            TRY(dev_alloc(&psi64[1], 3 * G));
+            TRY(dev_alloc(&b64, 3 * G));
+            TRY(dev_alloc(&r32, 3 * G));
+            TRY(dev_alloc(&e32, 3 * G));
+        } else {
+            TRY(dev_alloc(&psi32, 3 * G));
+        }
+        std::vector<float> dt = far_dir_table(p.K, pg.h);
+        TRY(dev_alloc(&dir, dt.size()));
+        PPPM_HIP_TRY(hipMemcpy(dir, dt.data(), dt.size() * sizeof(float), hipMemcpyHostToDevice));
+        TRY(mg.init((int)g, (int)g, (int)g, 3));
+        return PPPM_OK;
+    }
+
+    void release() {
+        vs.release();
+        es.release();
+        dev_free(spos);
+        dev_free(somg);
+        dev_free(cstart);
+        dev_free(cend);
+        dev_free(coff);
+        dev_free(scan_scr);
+        dev_free(d_strays);
+        if (h_strays) (void)hipHostFree(h_strays);
+        h_strays = nullptr;
+        dev_free(epos);
+        dev_free(eoth);
+        dev_free(wg);
+        dev_free(far);
+        dev_free(psi32);
+        dev_free(dir);
+        dev_free(psi64[0]);
+        dev_free(psi64[1]);
+        dev_free(b64);
+        dev_free(r32);
+        dev_free(e32);
+        mg.release();
+        if (own_stream && st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+
+    pppm_status ensure_vort(uint32_t nn) {
+        TRY(vs.ensure(nn));
+        if (nn > capN || !spos) {
+            dev_free(spos);
+            dev_free(somg);
+            uint32_t c = vs.cap;
+            TRY(dev_alloc(&spos, c));
+            TRY(dev_alloc(&somg, c));
+            capN = c;
+        }
+        return PPPM_OK;
+    }
+    pppm_status ensure_eval(uint32_t mm, bool other) {
+        TRY(es.ensure(mm));
+        if (mm > capM || !epos || (other && !eoth)) {
+            dev_free(epos);
+            dev_free(eoth);
+            uint32_t c = es.cap;
+            TRY(dev_alloc(&epos, c));
+            TRY(dev_alloc(&eoth, c));
+            capM = c;
+        }
+        return PPPM_OK;
+    }
+};
+
+// ---------------------------------------------------------------------------
+extern "C" void pppm_params_config_preset(pppm_params* p, uint32_t g) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->g = g;
+    p->h = 1.0 / (double)g;
+    p->origin[0] = p->origin[1] = p->origin[2] = 0.0;
+    p->K = 2;
+    p->kernel = PPPM_KERNEL_EXPCUBIC;
+    p->core_a = 1.5 * (double)(float)p->h;
+    p->mg_schedule = PPPM_MG_VCYCLE;
+    p->nu1 = 2;
+    p->nu2 = 2;
+    p->mg_omega = 1.0;
+    p->mg_tol = 1e-6;
+    p->mg_max_cycles = 40;
+    p->precision = PPPM_PREC_FP32;
+}
+
+extern "C" void pppm_params_reference_preset(pppm_params* p, uint32_t g, int32_t K) {
+    if (!p) return;
+    pppm_params_config_preset(p, g);
+    p->K = K;
+    p->core_a = 0.08;                       // GPU_kernel.cu:2306
+    p->mg_schedule = PPPM_MG_REF_FMG;       // MultiGrid.cpp:202-250
+    p->nu1 = p->nu2 = 4;                    // MultiGrid.cpp:149, 186
+    p->mg_omega = (double)1.43f;            // MultiGrid_kernel.cu:36
+    p->mg_max_cycles = 1;
+}
+
+extern "C" pppm_status pppm_reference_domain(const float* pos, uint32_t n, uint32_t g, double origin_out[3],
+                                             double* h_out) {
+    if (!pos || n == 0 || g == 0 || !origin_out || !h_out) FAIL(PPPM_ERR_INVALID, "reference_domain args");
+    double mx = pos[0], Mx = pos[0], my = pos[1], My = pos[1], mz = pos[2], Mz = pos[2];
+    for (uint32_t i = 1; i < n; i++) {
+        double x = pos[4 * (size_t)i], y = pos[4 * (size_t)i + 1], z = pos[4 * (size_t)i + 2];
+        if (x < mx) mx = x; else if (x > Mx) Mx = x;
+        if (y < my) my = y; else if (y > My) My = y;
+        if (z < mz) mz = z; else if (z > Mz) Mz = z;
+    }
+    double L = Mx - mx;
+    if (My - my > L) L = My - my;
+    if (Mz - mz > L) L = Mz - mz;
+    L *= 3;
+    origin_out[0] = (mx + 0.5 * (Mx - mx)) - 0.5 * L;
+    origin_out[1] = (my + 0.5 * (My - my)) - 0.5 * L;
+    origin_out[2] = (mz + 0.5 * (Mz - mz)) - 0.5 * L;
+    *h_out = L / (double)g;
+    if (!(L > 0)) FAIL(PPPM_ERR_INVALID, "degenerate particle extent");
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_create(const pppm_params* p, pppm_ctx** out) {
+    if (!out) FAIL(PPPM_ERR_INVALID, "null out");
+    *out = nullptr;
+    TRY(validate_params(p));
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) FAIL(PPPM_ERR_NO_DEVICE, "no HIP device");
+    pppm_ctx* c = new pppm_ctx();
+    pppm_status s = c->init(*p);
+    if (s != PPPM_OK) {
+        c->release();
+        delete c;
+        return s;
+    }
+    *out = c;
+    return PPPM_OK;
+}
+
+extern "C" void pppm_destroy(pppm_ctx* c) {
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    c->release();
+    delete c;
+}
+
+extern "C" pppm_status pppm_set_stream(pppm_ctx* c, void* s) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    PPPM_HIP_TRY(hipStreamSynchronize(c->st));
+    if (c->own_stream) (void)hipStreamDestroy(c->st);
+    c->st = (hipStream_t)s;
+    c->own_stream = false;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_get_stream(const pppm_ctx* c, void** s) {
+    if (!c || !s) FAIL(PPPM_ERR_INVALID, "null arg");
+    *s = (void*)c->st;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_get_params(const pppm_ctx* c, pppm_params* out) {
+    if (!c || !out) FAIL(PPPM_ERR_INVALID, "null arg");
+    *out = c->p;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_set_vortices(pppm_ctx* c, uint32_t n, const float* pos4, const void* wx, const void* wy,
+                                         const void* wz, uint32_t w_stride, int32_t w_dtype) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (n > 0 && (!pos4 || !wx || !wy || !wz)) FAIL(PPPM_ERR_INVALID, "null vortex array");
+    if (w_dtype != PPPM_F32 && w_dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "w_dtype");
+    if (w_stride == 0) FAIL(PPPM_ERR_INVALID, "w_stride");
+    if (n > (1u << 30)) FAIL(PPPM_ERR_INVALID, "too many vortices");
+    hipStream_t st = c->st;
+    TRY(c->ensure_vort(n));
+    c->n = n;
+    c->have_far = false;
+    TRY(launch_calc_keys((const float4*)pos4, n, c->pg.hash, c->vs.k0, nullptr, st));
+    TRY(radix_sort_pairs(c->vs.k0, c->vs.v0, c->vs.k1, c->vs.v1, n, key_bits_for(c->G), true, c->vs.scratch, &c->vs.ks,
+                         &c->vs.vs, st));
+    TRY(launch_cell_tables(c->vs.ks, n, (uint32_t)c->G, c->cstart, c->cend, st));
+    TRY(cell_offsets_from_tables(c->cstart, c->cend, (uint32_t)c->G, c->coff, c->scan_scr, st));
+    TRY(launch_fill_u32(c->d_strays, 1, 0u, st));
+    TRY(launch_reorder_vortices(c->vs.vs, c->vs.ks, (const float4*)pos4, wx, wy, wz, w_stride, w_dtype, n, c->pg,
+                                c->spos, c->somg, c->d_strays, st));
+    c->have_vort = true;
+    c->stats.n = n;
+    return PPPM_OK;
+}
+
+static pppm_status solve_mixed(pppm_ctx* c) {
+    hipStream_t st = c->st;
+    const int g = (int)c->g;
+    double* outn = c->mg.partials() + (size_t)MGHier<float>::kNormBlocks * 3 * 2;
+    double* hn = c->mg.host_norms();
+    auto read_rel = [&](double* rel) -> pppm_status {
+        PPPM_HIP_TRY(hipMemcpyAsync(hn, outn, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+        PPPM_HIP_TRY(hipStreamSynchronize(st));
+        double w = 0;
+        for (int k = 0; k < 3; k++) {
+            double rr = std::sqrt(hn[2 * k]), bb = std::sqrt(hn[2 * k + 1]);
+            rel[k] = bb > 0 ? rr / bb : rr;
+            if (rel[k] > w) w = rel[k];
+        }
+        rel[3] = w;
+        return PPPM_OK;
+    };
+    double rel[4];
+    c->psi_cur = 0;
+    TRY(launch_dc_step(nullptr, nullptr, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn, st));
+    TRY(read_rel(rel));
+    int outer = 0, inner = 0;
+    while (outer < c->p.mg_max_cycles && rel[3] > c->p.mg_tol) {
+        TRY(c->mg.fill(c->e32, 3 * c->G, st));
+        TRY(c->mg.vcycle(c->e32, c->r32, 1.0f, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st));
+        inner++;
+        int nxt = 1 - c->psi_cur;
+        TRY(launch_dc_step(c->psi64[c->psi_cur], c->e32, c->b64, c->psi64[nxt], c->r32, g, g, g, 3,
+                           c->mg.partials(), outn, st));
+        c->psi_cur = nxt;
+        outer++;
+        TRY(read_rel(rel));
+    }
+    c->stats.mg_cycles = inner;
+    c->stats.mg_outer = outer;
+    for (int k = 0; k < 3; k++) c->stats.mg_relres[k] = rel[k];
+    c->stats.mg_converged = rel[3] <= c->p.mg_tol;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_compute_far_field(pppm_ctx* c) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (!c->have_vort) FAIL(PPPM_ERR_STATE, "set_vortices first");
+    hipStream_t st = c->st;
+    const bool mixed = c->p.precision == PPPM_PREC_MIXED;
+    const double h = c->pg.h;
+    // strays decide whether the neighbourhood pass is needed
+    PPPM_HIP_TRY(hipMemcpyAsync(c->h_strays, c->d_strays, 4, hipMemcpyDeviceToHost, st));
+    TRY(launch_p2m(c->coff, c->spos, c->somg, c->g, c->pg.invh, c->wg, mixed ? c->b64 : nullptr, -h * h, st));
+    PPPM_HIP_TRY(hipStreamSynchronize(st));
+    c->stats.strays = *c->h_strays;
+    if (c->stats.strays)
+        TRY(launch_p2m_strays(c->coff, c->spos, c->somg, c->g, c->pg.invh, c->wg, mixed ? c->b64 : nullptr, -h * h,
+                              st));
+    const void* psi;
+    int pdt;
+    if (mixed) {
+        TRY(solve_mixed(c));
+        psi = c->psi64[c->psi_cur];
+        pdt = PPPM_F64;
+    } else {
+        const float bscale = (float)(-h * h);
+        TRY(c->mg.fill(c->psi32, 3 * c->G, st));
+        if (c->p.mg_schedule == PPPM_MG_REF_FMG) {
+            TRY(c->mg.ref_fmg(c->psi32, c->wg, bscale, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, st));
+            double rn[3], bn[3];
+            TRY(c->mg.norms(c->psi32, c->wg, bscale, rn, bn, st));
+            for (int k = 0; k < 3; k++) c->stats.mg_relres[k] = bn[k] > 0 ? rn[k] / bn[k] : rn[k];
+            c->stats.mg_cycles = 1;
+            c->stats.mg_converged = 1;
+        } else {
+            int cyc = 0;
+            pppm_status s = c->mg.solve(c->psi32, c->wg, bscale, c->p.nu1, c->p.nu2, (float)c->p.mg_omega,
+                                        c->p.mg_tol, c->p.mg_max_cycles, &cyc, c->stats.mg_relres, st);
+            if (s != PPPM_OK && s != PPPM_ERR_NOT_CONVERGED) return s;
+            c->stats.mg_cycles = cyc;
+            c->stats.mg_converged = s == PPPM_OK;
+        }
+        c->stats.mg_outer = 0;
+        psi = c->psi32;
+        pdt = PPPM_F32;
+    }
+    TRY(launch_curl_far_impl(psi, pdt, c->wg, c->far, c->dir, c->g, c->p.K, h, st));
+    c->have_far = true;
+    return PPPM_OK;
+}
+
+static pppm_status near_common(pppm_ctx* c, uint32_t m, const float4* epos, const float4* eoth,
+                               const uint32_t* eidx, void* ux, void* uy, void* uz, uint32_t u_stride,
+                               int32_t u_dtype) {
+    NearArgs a;
+    a.epos = epos;
+    a.eoth = eoth;
+    a.eidx = eidx;
+    a.m = m;
+    a.off = c->coff;
+    a.spos = c->spos;
+    a.somg = c->somg;
+    a.far = c->far;
+    a.g = c->g;
+    a.K = c->p.K;
+    a.kernel = c->p.kernel;
+    a.invh = c->pg.invh;
+    a.core_a = c->p.core_a;
+    a.ux = ux;
+    a.uy = uy;
+    a.uz = uz;
+    a.u_stride = u_stride;
+    a.u_dtype = u_dtype;
+    return launch_m2p_near(a, c->st);
+}
+
+extern "C" pppm_status pppm_evaluate_self(pppm_ctx* c, void* ux, void* uy, void* uz, uint32_t u_stride,
+                                          int32_t u_dtype) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (!c->have_far) FAIL(PPPM_ERR_STATE, "compute_far_field first");
+    if (c->n > 0 && (!ux || !uy || !uz)) FAIL(PPPM_ERR_INVALID, "null output");
+    if (u_dtype != PPPM_F32 && u_dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "u_dtype");
+    if (u_stride == 0) FAIL(PPPM_ERR_INVALID, "u_stride");
+    c->stats.m = c->n;
+    return near_common(c, c->n, c->spos, nullptr, c->vs.vs, ux, uy, uz, u_stride, u_dtype);
+}
+
+extern "C" pppm_status pppm_evaluate(pppm_ctx* c, uint32_t m, const float* eval_pos4, const float* other_end4,
+                                     void* ux, void* uy, void* uz, uint32_t u_stride, int32_t u_dtype) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (!c->have_far) FAIL(PPPM_ERR_STATE, "compute_far_field first");
+    if (m > 0 && (!eval_pos4 || !ux || !uy || !uz)) FAIL(PPPM_ERR_INVALID, "null array");
+    if (u_dtype != PPPM_F32 && u_dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "u_dtype");
+    if (u_stride == 0) FAIL(PPPM_ERR_INVALID, "u_stride");
+    if (m > (1u << 30)) FAIL(PPPM_ERR_INVALID, "too many points");
+    hipStream_t st = c->st;
+    TRY(c->ensure_eval(m, other_end4 != nullptr));
+    TRY(launch_calc_keys((const float4*)eval_pos4, m, c->pg.hash, c->es.k0, nullptr, st));
+    TRY(radix_sort_pairs(c->es.k0, c->es.v0, c->es.k1, c->es.v1, m, key_bits_for(c->G), true, c->es.scratch,
+                         &c->es.ks, &c->es.vs, st));
+    TRY(launch_reorder_eval(c->es.vs, (const float4*)eval_pos4, (const float4*)other_end4, m, c->pg, c->epos,
+                            other_end4 ? c->eoth : nullptr, st));
+    c->stats.m = m;
+    return near_common(c, m, c->epos, other_end4 ? c->eoth : nullptr, c->es.vs, ux, uy, uz, u_stride, u_dtype);
+}
+
+extern "C" pppm_status pppm_solve_self(pppm_ctx* c, uint32_t n, const float* pos4, const void* wx, const void* wy,
+                                       const void* wz, uint32_t w_stride, int32_t w_dtype, void* ux, void* uy,
+                                       void* uz, uint32_t u_stride, int32_t u_dtype) {
+    TRY(pppm_set_vortices(c, n, pos4, wx, wy, wz, w_stride, w_dtype));
+    TRY(pppm_compute_far_field(c));
+    return pppm_evaluate_self(c, ux, uy, uz, u_stride, u_dtype);
+}
+
+extern "C" pppm_status pppm_get_stats(const pppm_ctx* c, pppm_stats* out) {
+    if (!c || !out) FAIL(PPPM_ERR_INVALID, "null arg");
+    *out = c->stats;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_get_sorted_vortices(pppm_ctx* c, const float** pos4, const uint32_t** sorted_index,
+                                                const uint32_t** cell_start, const uint32_t** cell_end) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (!c->have_vort) FAIL(PPPM_ERR_STATE, "set_vortices first");
+    if (pos4) *pos4 = (const float*)c->spos;
+    if (sorted_index) *sorted_index = c->vs.vs;
+    if (cell_start) *cell_start = c->cstart;
+    if (cell_end) *cell_end = c->cend;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_get_grid(pppm_ctx* c, int32_t which, int32_t comp, const void** ptr, int32_t* dtype) {
+    if (!c || !ptr || !dtype || comp < 0 || comp > 2) FAIL(PPPM_ERR_INVALID, "get_grid args");
+    if (!c->have_far) FAIL(PPPM_ERR_STATE, "compute_far_field first");
+    switch (which) {
+        case 0:
+            *ptr = c->wg + comp * c->G;
+            *dtype = PPPM_F32;
+            break;
+        case 1:
+            if (c->p.precision == PPPM_PREC_MIXED) {
+                *ptr = c->psi64[c->psi_cur] + comp * c->G;
+                *dtype = PPPM_F64;
+            } else {
+                *ptr = c->psi32 + comp * c->G;
+                *dtype = PPPM_F32;
+            }
+            break;
+        case 2:
+            *ptr = c->far + comp * c->G;
+            *dtype = PPPM_F32;
+            break;
+        default:
+            FAIL(PPPM_ERR_INVALID, "which");
+    }
+    return PPPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device memory
+extern "C" pppm_status pppm_device_count(int32_t* n) {
+    int d = 0;
+    if (hipGetDeviceCount(&d) != hipSuccess) d = 0;
+    (void)hipGetLastError();
+    if (n) *n = d;
+    return d > 0 ? PPPM_OK : PPPM_ERR_NO_DEVICE;
+}
+extern "C" pppm_status pppm_malloc(void** ptr, size_t bytes) {
+    if (!ptr) FAIL(PPPM_ERR_INVALID, "null ptr");
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        *ptr = nullptr;
+        (void)hipGetLastError();
+        FAIL(PPPM_ERR_NOMEM, "hipMalloc failed");
+    }
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_free(void* ptr) {
+    if (ptr) PPPM_HIP_TRY(hipFree(ptr));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    if (bytes) PPPM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    if (bytes) PPPM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    if (bytes) PPPM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_memset(void* dst, int value, size_t bytes) {
+    if (bytes) PPPM_HIP_TRY(hipMemset(dst, value, bytes));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_synchronize(void) {
+    PPPM_HIP_TRY(hipDeviceSynchronize());
+    return PPPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timers
+struct pppm_timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+extern "C" pppm_status pppm_timer_create(pppm_timer** out) {
+    if (!out) FAIL(PPPM_ERR_INVALID, "null out");
+    pppm_timer* t = new pppm_timer();
+    if (hipEventCreate(&t->a) != hipSuccess || hipEventCreate(&t->b) != hipSuccess) {
+        delete t;
+        FAIL(PPPM_ERR_HIP, "event create failed");
+    }
+    *out = t;
+    return PPPM_OK;
+}
+extern "C" void pppm_timer_destroy(pppm_timer* t) {
+    if (!t) return;
+    (void)hipEventDestroy(t->a);
+    (void)hipEventDestroy(t->b);
+    delete t;
+}
+extern "C" pppm_status pppm_timer_start(pppm_timer* t, void* s) {
+    PPPM_HIP_TRY(hipEventRecord(t->a, (hipStream_t)s));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_timer_stop(pppm_timer* t, void* s) {
+    PPPM_HIP_TRY(hipEventRecord(t->b, (hipStream_t)s));
+    return PPPM_OK;
+}
+extern "C" pppm_status pppm_timer_elapsed_ms(pppm_timer* t, float* ms) {
+    PPPM_HIP_TRY(hipEventSynchronize(t->b));
+    PPPM_HIP_TRY(hipEventElapsedTime(ms, t->a, t->b));
+    return PPPM_OK;
+}

@@ -1,0 +1,90 @@
+// Curl of the stream function fused with the far-field correction
+// (GPU_kernel.cu:1750-1805 compute_curl_kernel, 2487-2564 Step1, 1976-2001
+// compute_diag_Psi_kernel, 2566-2630 subtract_diagonal_grid_kernel, host
+// orchestration 2634-2734):
+//   far[c] = curl(psi)[c]
+//          - sum_{D != 0, |D|inf <= K, c+D in grid} dir[D] x (w[c+D] h^3)
+//          - curl(0.250580605913965 h^2 w)[c]
+// with clamp-to-edge central differences.  The reference's D2H/H2D stash of
+// grid_U and its per-call host table upload are gone: dir[] is built once per
+// (K, h) and read with wave-uniform (scalar) loads.
+#include "pppm_internal.hpp"
+
+namespace pppm {
+
+template <typename TP>
+__device__ inline double ld(const TP* p, size_t i) {
+    return (double)p[i];
+}
+
+template <typename TP, typename TC>
+__global__ void __launch_bounds__(256) k_curl_far(const TP* __restrict__ psi, const float* __restrict__ wg,
+                                                  float* __restrict__ far, const float* __restrict__ dir, uint32_t g,
+                                                  int K, float h3, TC half_invh, float cdiag_half_invh) {
+    const uint32_t G = g * g * g;
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G) return;
+    const int gi = (int)g;
+    const int i = (int)(c % g), j = (int)((c / g) % g), k = (int)(c / (g * g));
+    const int im = i > 0 ? i - 1 : 0, ip = i < gi - 1 ? i + 1 : gi - 1;
+    const int jm = j > 0 ? j - 1 : 0, jp = j < gi - 1 ? j + 1 : gi - 1;
+    const int km = k > 0 ? k - 1 : 0, kp = k < gi - 1 ? k + 1 : gi - 1;
+    auto at = [&](int a, int b, int cc) { return ((size_t)cc * g + (size_t)b) * g + (size_t)a; };
+    const TP* Px = psi;
+    const TP* Py = psi + G;
+    const TP* Pz = psi + 2 * (size_t)G;
+    // curl(psi), GPU_kernel.cu:1770-1786 (evaluation order kept)
+    TC ux = half_invh * ((TC)Pz[at(i, jp, k)] - (TC)Pz[at(i, jm, k)] + (TC)Py[at(i, j, km)] - (TC)Py[at(i, j, kp)]);
+    TC uy = half_invh * ((TC)Px[at(i, j, kp)] - (TC)Px[at(i, j, km)] + (TC)Pz[at(im, j, k)] - (TC)Pz[at(ip, j, k)]);
+    TC uz = half_invh * ((TC)Py[at(ip, j, k)] - (TC)Py[at(im, j, k)] + (TC)Px[at(i, jm, k)] - (TC)Px[at(i, jp, k)]);
+    const float* Wx = wg;
+    const float* Wy = wg + G;
+    const float* Wz = wg + 2 * (size_t)G;
+    // diagonal term: curl of 0.2505806 h^2 w with the same clamped stencil
+    float dx = cdiag_half_invh * (Wz[at(i, jp, k)] - Wz[at(i, jm, k)] + Wy[at(i, j, km)] - Wy[at(i, j, kp)]);
+    float dy = cdiag_half_invh * (Wx[at(i, j, kp)] - Wx[at(i, j, km)] + Wz[at(im, j, k)] - Wz[at(ip, j, k)]);
+    float dz = cdiag_half_invh * (Wy[at(ip, j, k)] - Wy[at(im, j, k)] + Wx[at(i, jm, k)] - Wx[at(i, jp, k)]);
+    // local correction over the (2K+1)^3 block
+    const int W = 2 * K + 1;
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    for (int a = -K; a <= K; a++) {
+        const int kk = k + a;
+        if (kk < 0 || kk >= gi) continue;
+        for (int b = -K; b <= K; b++) {
+            const int jj = j + b;
+            if (jj < 0 || jj >= gi) continue;
+            const float* drow = dir + 3 * (size_t)(((a + K) * W + (b + K)) * W);
+            for (int e = -K; e <= K; e++) {
+                const int ii = i + e;
+                if (ii < 0 || ii >= gi) continue;
+                const float d0 = drow[3 * (e + K)], d1 = drow[3 * (e + K) + 1], d2 = drow[3 * (e + K) + 2];
+                const size_t q = at(ii, jj, kk);
+                const float ox = Wx[q] * h3, oy = Wy[q] * h3, oz = Wz[q] * h3;
+                vx += d1 * oz - d2 * oy;
+                vy += d2 * ox - d0 * oz;
+                vz += d0 * oy - d1 * ox;
+            }
+        }
+    }
+    far[c] = (float)((double)ux - (double)vx - (double)dx);
+    far[(size_t)G + c] = (float)((double)uy - (double)vy - (double)dy);
+    far[2 * (size_t)G + c] = (float)((double)uz - (double)vz - (double)dz);
+}
+
+pppm_status launch_curl_far_impl(const void* psi, int psi_dtype, const float* wgrid, float* far, const float* dir,
+                                 uint32_t g, int K, double h, hipStream_t st) {
+    const uint32_t G = g * g * g;
+    const float h3 = (float)(h * h * h);
+    const double half_invh = 0.5 * (1.0 / h);
+    const float cd = (float)(kRefDiagPsi * h * h * half_invh);
+    if (psi_dtype == PPPM_F64)
+        hipLaunchKernelGGL((k_curl_far<double, double>), dim3(div_up(G, 256)), dim3(256), 0, st, (const double*)psi,
+                           wgrid, far, dir, g, K, h3, half_invh, cd);
+    else
+        hipLaunchKernelGGL((k_curl_far<float, float>), dim3(div_up(G, 256)), dim3(256), 0, st, (const float*)psi,
+                           wgrid, far, dir, g, K, h3, (float)half_invh, cd);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+}  // namespace pppm

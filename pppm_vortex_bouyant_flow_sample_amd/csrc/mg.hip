@@ -1,0 +1,706 @@
+// Multigrid kernels and hierarchy driver (see mg.hpp).
+//   smoother      MultiGrid_kernel.cu:722-820, Multigrid.cu:202-239 (red/black SOR)
+//   residual      MultiGrid_kernel.cu:821-871
+//   restriction   MultiGrid_kernel.cu:873-910 (fused with the residual here)
+//   prolongation  MultiGrid_kernel.cu:912-934 (piecewise constant) + trilinear
+//   coarse solve  MultiGrid_kernel.cu:422-498 replaced by an exact eigenbasis solve
+//   schedules     MultiGrid.cpp:128-250
+#include "mg.hpp"
+#include <cmath>
+#include <cstring>
+
+namespace pppm {
+
+double mg_beta(int level) {
+    double s = std::ldexp(1.0, -level);
+    return -(1.0 - s) / (1.0 + s);
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T nb_sum(const T* __restrict__ x, size_t t, int i, int j, int k, int nx, int ny, int nz,
+                           int& nout) {
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    T lv = i > 0 ? x[t - 1] : (T)0;
+    T rv = i < nx - 1 ? x[t + 1] : (T)0;
+    T uv = j < ny - 1 ? x[t + sy] : (T)0;
+    T dv = j > 0 ? x[t - sy] : (T)0;
+    T fv = k > 0 ? x[t - sz] : (T)0;
+    T bv = k < nz - 1 ? x[t + sz] : (T)0;
+    nout = (i == 0) + (i == nx - 1) + (j == 0) + (j == ny - 1) + (k == 0) + (k == nz - 1);
+    T r = lv + rv;
+    r += uv + dv;
+    r += fv + bv;
+    return r;
+}
+
+// One colour of red-black SOR.  color 1 = (i+j+k) odd (swept first, as the
+// reference's "red" kernel), 0 = even.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rbgs(T* __restrict__ x, const T* __restrict__ b, T bscale, int nx, int ny,
+                                              int nz, size_t G, int color, T omega, T beta) {
+    const int i2 = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int comp = blockIdx.z / nz, k = blockIdx.z - comp * nz;
+    if (j >= ny) return;
+    const int i = 2 * i2 + ((j + k + color) & 1);
+    if (i >= nx) return;
+    T* xc = x + (size_t)comp * G;
+    const size_t t = ((size_t)k * ny + j) * nx + i;
+    int nout;
+    T r = nb_sum(xc, t, i, j, k, nx, ny, nz, nout);
+    r -= bscale * b[(size_t)comp * G + t];
+    T xn;
+    if (beta == (T)0 || nout == 0)
+        xn = ((T)1 - omega) * xc[t] + omega * r * (T)(1.0 / 6.0);
+    else
+        xn = ((T)1 - omega) * xc[t] + omega * r / ((T)6 - beta * (T)nout);
+    xc[t] = xn;
+}
+
+template <typename T>
+__device__ inline T residual_at(const T* __restrict__ x, const T* __restrict__ b, T bscale, int i, int j, int k,
+                                int nx, int ny, int nz, T beta) {
+    const size_t t = ((size_t)k * ny + j) * nx + i;
+    int nout;
+    T s = nb_sum(x, t, i, j, k, nx, ny, nz, nout);
+    s -= (T)6 * x[t];
+    if (beta != (T)0 && nout) s += beta * (T)nout * x[t];
+    return bscale * b[t] - s;
+}
+
+// r = b - A x at the 8 children of each coarse cell, b_c = 0.5 * sum (children
+// order as MultiGrid_kernel.cu:897-904).
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual_restrict(const T* __restrict__ x, const T* __restrict__ b, T bscale,
+                                                           T* __restrict__ bn, int nx, int ny, int nz, int cx,
+                                                           int cy, int cz, size_t G, size_t Gc, T beta) {
+    const int ci = blockIdx.x * 64 + threadIdx.x;
+    const int cj = blockIdx.y * 4 + threadIdx.y;
+    const int comp = blockIdx.z / cz, ck = blockIdx.z - comp * cz;
+    if (ci >= cx || cj >= cy) return;
+    const T* xc = x + (size_t)comp * G;
+    const T* bc = b + (size_t)comp * G;
+    const int i = 2 * ci, j = 2 * cj, k = 2 * ck;
+    // children beyond a fine dimension of 1 do not exist
+    const bool hi = i + 1 < nx, hj = j + 1 < ny, hk = k + 1 < nz;
+    T r = 0;
+    r += residual_at(xc, bc, bscale, i, j, k, nx, ny, nz, beta);
+    if (hi) r += residual_at(xc, bc, bscale, i + 1, j, k, nx, ny, nz, beta);
+    if (hj) r += residual_at(xc, bc, bscale, i, j + 1, k, nx, ny, nz, beta);
+    if (hi && hj) r += residual_at(xc, bc, bscale, i + 1, j + 1, k, nx, ny, nz, beta);
+    if (hk) r += residual_at(xc, bc, bscale, i, j, k + 1, nx, ny, nz, beta);
+    if (hi && hk) r += residual_at(xc, bc, bscale, i + 1, j, k + 1, nx, ny, nz, beta);
+    if (hj && hk) r += residual_at(xc, bc, bscale, i, j + 1, k + 1, nx, ny, nz, beta);
+    if (hi && hj && hk) r += residual_at(xc, bc, bscale, i + 1, j + 1, k + 1, nx, ny, nz, beta);
+    bn[(size_t)comp * Gc + ((size_t)ck * cy + cj) * cx + ci] = r * (T)0.5;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(T* __restrict__ x, const T* __restrict__ xc, int nx, int ny, int nz,
+                                                 int cx, int cy, int cz, size_t G, size_t Gc, int mode, T betac) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int comp = blockIdx.z / nz, k = blockIdx.z - comp * nz;
+    if (i >= nx || j >= ny) return;
+    const T* c = xc + (size_t)comp * Gc;
+    int pi = min(i / 2, cx - 1), pj = min(j / 2, cy - 1), pk = min(k / 2, cz - 1);
+    T v;
+    if (mode == PROLONG_CONSTANT) {
+        v = c[((size_t)pk * cy + pj) * cx + pi];
+    } else {
+        // 1-D weights 3/4 (parent) and 1/4 (neighbour on the fine cell's side);
+        // a neighbour outside the coarse grid is betac * parent value
+        int ni = (i & 1) ? pi + 1 : pi - 1, nj = (j & 1) ? pj + 1 : pj - 1, nk = (k & 1) ? pk + 1 : pk - 1;
+        T wi = nx == 1 ? (T)0 : (T)0.25, wj = ny == 1 ? (T)0 : (T)0.25, wk = nz == 1 ? (T)0 : (T)0.25;
+        T fi = (T)1, fj = (T)1, fk = (T)1;
+        if (ni < 0 || ni > cx - 1) { ni = pi; fi = betac; }
+        if (nj < 0 || nj > cy - 1) { nj = pj; fj = betac; }
+        if (nk < 0 || nk > cz - 1) { nk = pk; fk = betac; }
+        auto C = [&](int a, int bb, int cc) { return c[((size_t)cc * cy + bb) * cx + a]; };
+        T p0 = ((T)1 - wi) * C(pi, pj, pk) + wi * fi * C(ni, pj, pk);
+        T p1 = ((T)1 - wi) * C(pi, nj, pk) + wi * fi * C(ni, nj, pk);
+        T p2 = ((T)1 - wi) * C(pi, pj, nk) + wi * fi * C(ni, pj, nk);
+        T p3 = ((T)1 - wi) * C(pi, nj, nk) + wi * fi * C(ni, nj, nk);
+        T q0 = ((T)1 - wj) * p0 + wj * fj * p1;
+        T q1 = ((T)1 - wj) * p2 + wj * fj * p3;
+        v = ((T)1 - wk) * q0 + wk * fk * q1;
+    }
+    x[(size_t)comp * G + ((size_t)k * ny + j) * nx + i] += v;
+}
+
+// Exact solve of the <= 512-cell coarsest level in the separable eigenbasis.
+// One workgroup per component; FP64 throughout.
+template <typename T>
+__global__ void __launch_bounds__(512) k_coarse(T* __restrict__ x, const T* __restrict__ b, T bscale, int nx, int ny,
+                                                int nz, size_t G, const double* __restrict__ tab) {
+    __shared__ double v[512];
+    __shared__ double w[512];
+    const int comp = blockIdx.x;
+    const int n = nx * ny * nz;
+    const int t = threadIdx.x;
+    const double* Ex = tab;
+    const double* Ey = Ex + nx * nx;
+    const double* Ez = Ey + ny * ny;
+    const double* lx = Ez + nz * nz;
+    const double* ly = lx + nx;
+    const double* lz = ly + ny;
+    const int i = t % nx, j = (t / nx) % ny, k = t / (nx * ny);
+    if (t < n) v[t] = (double)bscale * (double)b[(size_t)comp * G + t];
+    __syncthreads();
+    // forward: v <- E v along x, y, z
+    if (t < n) {
+        double s = 0;
+        for (int q = 0; q < nx; q++) s += Ex[i * nx + q] * v[(k * ny + j) * nx + q];
+        w[t] = s;
+    }
+    __syncthreads();
+    if (t < n) {
+        double s = 0;
+        for (int q = 0; q < ny; q++) s += Ey[j * ny + q] * w[(k * ny + q) * nx + i];
+        v[t] = s;
+    }
+    __syncthreads();
+    if (t < n) {
+        double s = 0;
+        for (int q = 0; q < nz; q++) s += Ez[k * nz + q] * v[(q * ny + j) * nx + i];
+        w[t] = s / (lx[i] + ly[j] + lz[k]);
+    }
+    __syncthreads();
+    // backward: v <- E^T v along x, y, z
+    if (t < n) {
+        double s = 0;
+        for (int q = 0; q < nx; q++) s += Ex[q * nx + i] * w[(k * ny + j) * nx + q];
+        v[t] = s;
+    }
+    __syncthreads();
+    if (t < n) {
+        double s = 0;
+        for (int q = 0; q < ny; q++) s += Ey[q * ny + j] * v[(k * ny + q) * nx + i];
+        w[t] = s;
+    }
+    __syncthreads();
+    if (t < n) {
+        double s = 0;
+        for (int q = 0; q < nz; q++) s += Ez[q * nz + k] * w[(q * ny + j) * nx + i];
+        x[(size_t)comp * G + t] = (T)s;
+    }
+}
+
+template <typename T>
+__global__ void k_fill_zero(T* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = (T)0;
+}
+
+__device__ inline double block_sum_256(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w2 = 0; w2 < (int)(blockDim.x >> 6); w2++) s += lds[w2];
+    __syncthreads();
+    return s;
+}
+
+// Residual and right-hand-side norms, block partials in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256) k_resnorm(const T* __restrict__ x, const T* __restrict__ b, T bscale, int nx,
+                                                 int ny, int nz, size_t G, double* __restrict__ part) {
+    __shared__ double lds[4];
+    const int comp = blockIdx.y;
+    const T* xc = x + (size_t)comp * G;
+    const T* bc = b + (size_t)comp * G;
+    double rr = 0, bb = 0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < G; t += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
+        T r = residual_at(xc, bc, bscale, i, j, k, nx, ny, nz, (T)0);
+        double bv = (double)bscale * (double)bc[t];
+        rr += (double)r * (double)r;
+        bb += bv * bv;
+    }
+    double s1 = block_sum_256(rr, lds);
+    double s2 = block_sum_256(bb, lds);
+    if (threadIdx.x == 0) {
+        part[((size_t)comp * gridDim.x + blockIdx.x) * 2] = s1;
+        part[((size_t)comp * gridDim.x + blockIdx.x) * 2 + 1] = s2;
+    }
+}
+
+// out[2c], out[2c+1] = sums of the partials of component c (fixed order).
+__global__ void __launch_bounds__(256) k_reduce_partials(const double* __restrict__ part, int nblk,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[4];
+    const int comp = blockIdx.x;
+    double a = 0, b2 = 0;
+    for (int q = threadIdx.x; q < nblk; q += blockDim.x) {
+        a += part[((size_t)comp * nblk + q) * 2];
+        b2 += part[((size_t)comp * nblk + q) * 2 + 1];
+    }
+    double s1 = block_sum_256(a, lds);
+    double s2 = block_sum_256(b2, lds);
+    if (threadIdx.x == 0) {
+        out[2 * comp] = s1;
+        out[2 * comp + 1] = s2;
+    }
+}
+
+// FP64 defect-correction step (see mg.hpp).
+__global__ void __launch_bounds__(256) k_dc_step(const double* __restrict__ xin, const float* __restrict__ e,
+                                                 const double* __restrict__ b64, double* __restrict__ xout,
+                                                 float* __restrict__ r32, int nx, int ny, int nz, size_t G,
+                                                 double* __restrict__ part) {
+    __shared__ double lds[4];
+    const int comp = blockIdx.y;
+    const double* xi = xin ? xin + (size_t)comp * G : nullptr;
+    const float* ec = e ? e + (size_t)comp * G : nullptr;
+    double rr = 0, bb = 0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < G; t += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
+        auto X = [&](size_t q) { return (xi ? xi[q] : 0.0) + (ec ? (double)ec[q] : 0.0); };
+        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+        double xc = X(t);
+        double lv = i > 0 ? X(t - 1) : 0.0;
+        double rv = i < nx - 1 ? X(t + 1) : 0.0;
+        double uv = j < ny - 1 ? X(t + sy) : 0.0;
+        double dv = j > 0 ? X(t - sy) : 0.0;
+        double fv = k > 0 ? X(t - sz) : 0.0;
+        double bv = k < nz - 1 ? X(t + sz) : 0.0;
+        double s = lv + rv;
+        s += uv + dv;
+        s += fv + bv;
+        s -= 6.0 * xc;
+        double bval = b64[(size_t)comp * G + t];
+        double r = bval - s;
+        xout[(size_t)comp * G + t] = xc;
+        r32[(size_t)comp * G + t] = (float)r;
+        rr += r * r;
+        bb += bval * bval;
+    }
+    double s1 = block_sum_256(rr, lds);
+    double s2 = block_sum_256(bb, lds);
+    if (threadIdx.x == 0) {
+        part[((size_t)comp * gridDim.x + blockIdx.x) * 2] = s1;
+        part[((size_t)comp * gridDim.x + blockIdx.x) * 2 + 1] = s2;
+    }
+}
+
+pppm_status launch_dc_step(const double* x_in, const float* e, const double* b64, double* x_out, float* r32, int nx,
+                           int ny, int nz, int ncomp, double* partials, double* out, hipStream_t st) {
+    const size_t G = (size_t)nx * ny * nz;
+    const int nblk = MGHier<float>::kNormBlocks;
+    hipLaunchKernelGGL(k_dc_step, dim3(nblk, ncomp), dim3(256), 0, st, x_in, e, b64, x_out, r32, nx, ny, nz, G,
+                       partials);
+    PPPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(256), 0, st, partials, nblk, out);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+static void dst_basis(int n, double* E, double* lam) {
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < n; k++) {
+        double s = std::sin(pi * (k + 1) / (2.0 * (n + 1)));
+        lam[k] = -4.0 * s * s;
+        for (int i = 0; i < n; i++) E[k * n + i] = std::sqrt(2.0 / (n + 1)) * std::sin(pi * (double)(k + 1) * (i + 1) / (n + 1));
+    }
+}
+
+// cyclic Jacobi eigen-decomposition of a symmetric matrix (rows of E = vectors)
+static void eig_sym(int n, std::vector<double>& A, double* E, double* lam) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) E[i * n + j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) off += A[p * n + q] * A[p * n + q];
+        if (off < 1e-300) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int i = 0; i < n; i++) {
+                    double aip = A[i * n + p], aiq = A[i * n + q];
+                    A[i * n + p] = c * aip - s * aiq;
+                    A[i * n + q] = s * aip + c * aiq;
+                }
+                for (int i = 0; i < n; i++) {
+                    double api = A[p * n + i], aqi = A[q * n + i];
+                    A[p * n + i] = c * api - s * aqi;
+                    A[q * n + i] = s * api + c * aqi;
+                }
+                for (int i = 0; i < n; i++) {
+                    double ep = E[p * n + i], eq = E[q * n + i];
+                    E[p * n + i] = c * ep - s * eq;
+                    E[q * n + i] = s * ep + c * eq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) lam[i] = A[i * n + i];
+}
+
+static void axis_basis(int n, double beta, double* E, double* lam) {
+    if (beta == 0.0) {
+        dst_basis(n, E, lam);
+        return;
+    }
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        A[i * n + i] = -2.0;
+        if (i > 0) A[i * n + i - 1] = 1.0;
+        if (i < n - 1) A[i * n + i + 1] = 1.0;
+    }
+    A[0] += beta;
+    A[(size_t)(n - 1) * n + (n - 1)] += beta;
+    eig_sym(n, A, E, lam);
+}
+
+std::vector<double> coarse_tables(int nx, int ny, int nz, double beta) {
+    std::vector<double> t((size_t)nx * nx + ny * ny + nz * nz + nx + ny + nz);
+    double* Ex = t.data();
+    double* Ey = Ex + nx * nx;
+    double* Ez = Ey + ny * ny;
+    double* lx = Ez + nz * nz;
+    double* ly = lx + nx;
+    double* lz = ly + ny;
+    axis_basis(nx, beta, Ex, lx);
+    axis_basis(ny, beta, Ey, ly);
+    axis_basis(nz, beta, Ez, lz);
+    return t;
+}
+
+template <typename T>
+pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
+    release();
+    if (nx < 1 || ny < 1 || nz < 1 || ncomp < 1 || ncomp > 3) return PPPM_ERR_INVALID;
+    ncomp_ = ncomp;
+    // MultiGrid.h:115-127 (m_ComputeLevels), 93-98
+    L_ = 0;
+    int x = nx, y = ny, z = nz;
+    d_[0][0] = x;
+    d_[0][1] = y;
+    d_[0][2] = z;
+    while ((long)x * y * z > 512 && L_ + 1 < 16) {
+        x = x / 2 > 0 ? x / 2 : 1;
+        y = y / 2 > 0 ? y / 2 : 1;
+        z = z / 2 > 0 ? z / 2 : 1;
+        L_++;
+        d_[L_][0] = x;
+        d_[L_][1] = y;
+        d_[L_][2] = z;
+    }
+    L_ += 1;
+    x_.assign(L_, nullptr);
+    b_.assign(L_, nullptr);
+    bytes_ = 0;
+    for (int l = 1; l < L_; l++) {
+        size_t n = cells(l) * ncomp_;
+        if (hipMalloc(&x_[l], n * sizeof(T)) != hipSuccess || hipMalloc(&b_[l], n * sizeof(T)) != hipSuccess) {
+            release();
+            return PPPM_ERR_NOMEM;
+        }
+        bytes_ += 2 * n * sizeof(T);
+    }
+    const int* dc = d_[L_ - 1];
+    std::vector<double> tc = coarse_tables(dc[0], dc[1], dc[2], mg_beta(L_ - 1));
+    std::vector<double> tr = coarse_tables(dc[0], dc[1], dc[2], 0.0);
+    if (hipMalloc(&tab_cons_, tc.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&tab_ref_, tr.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&part_, (size_t)kNormBlocks * 3 * 2 * sizeof(double) + 6 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&hnorm_, 8 * sizeof(double)) != hipSuccess) {
+        release();
+        return PPPM_ERR_NOMEM;
+    }
+    PPPM_HIP_TRY(hipMemcpy(tab_cons_, tc.data(), tc.size() * sizeof(double), hipMemcpyHostToDevice));
+    PPPM_HIP_TRY(hipMemcpy(tab_ref_, tr.data(), tr.size() * sizeof(double), hipMemcpyHostToDevice));
+    return PPPM_OK;
+}
+
+template <typename T>
+void MGHier<T>::release() {
+    for (T* p : x_)
+        if (p) (void)hipFree(p);
+    for (T* p : b_)
+        if (p) (void)hipFree(p);
+    x_.clear();
+    b_.clear();
+    if (tab_cons_) (void)hipFree(tab_cons_);
+    if (tab_ref_) (void)hipFree(tab_ref_);
+    if (part_) (void)hipFree(part_);
+    if (hnorm_) (void)hipHostFree(hnorm_);
+    tab_cons_ = tab_ref_ = part_ = hnorm_ = nullptr;
+    L_ = 0;
+}
+
+template <typename T>
+pppm_status MGHier<T>::fill(T* p, size_t n, hipStream_t st) {
+    if (n == 0) return PPPM_OK;
+    size_t blocks = div_up64(n, 256);
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3((uint32_t)blocks), dim3(256), 0, st, p, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::smooth(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st) {
+    const int nx = d_[l][0], ny = d_[l][1], nz = d_[l][2];
+    dim3 blk(64, 4), grid(div_up(div_up(nx, 2), 64), div_up(ny, 4), nz * ncomp_);
+    for (int it = 0; it < iters; it++)
+        for (int color = 1; color >= 0; color--) {
+            hipLaunchKernelGGL(k_rbgs<T>, grid, blk, 0, st, x, b, bscale, nx, ny, nz, cells(l), color, omega, beta);
+            PPPM_LAUNCH_CHECK();
+        }
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::residual_restrict(int l, const T* x, const T* b, T bscale, T beta, T* bnext, hipStream_t st) {
+    const int* f = d_[l];
+    const int* c = d_[l + 1];
+    dim3 blk(64, 4), grid(div_up(c[0], 64), div_up(c[1], 4), c[2] * ncomp_);
+    hipLaunchKernelGGL(k_residual_restrict<T>, grid, blk, 0, st, x, b, bscale, bnext, f[0], f[1], f[2], c[0], c[1],
+                       c[2], cells(l), cells(l + 1), beta);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::prolong(int l, T* x, const T* xc, int mode, T betac, hipStream_t st) {
+    const int* f = d_[l];
+    const int* c = d_[l + 1];
+    dim3 blk(64, 4), grid(div_up(f[0], 64), div_up(f[1], 4), f[2] * ncomp_);
+    hipLaunchKernelGGL(k_prolong<T>, grid, blk, 0, st, x, xc, f[0], f[1], f[2], c[0], c[1], c[2], cells(l),
+                       cells(l + 1), mode, betac);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::coarse(T* x, const T* b, T bscale, bool consistent, hipStream_t st) {
+    const int* c = d_[L_ - 1];
+    hipLaunchKernelGGL(k_coarse<T>, dim3(ncomp_), dim3(512), 0, st, x, b, bscale, c[0], c[1], c[2], cells(L_ - 1),
+                       consistent ? tab_cons_ : tab_ref_);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+// MultiGrid.cpp:128-201 (m_Vcycle)
+template <typename T>
+static pppm_status vcycle_from(MGHier<T>& m, std::vector<T*>& X, std::vector<T*>& B, int l0, T bscale0, int nu1,
+                               int nu2, T omega, bool consistent, hipStream_t st) {
+    const int L = m.levels();
+    pppm_status s;
+    for (int l = l0; l < L - 1; l++) {
+        T beta = consistent ? (T)mg_beta(l) : (T)0;
+        T bs = l == 0 ? bscale0 : (T)1;
+        if (l > l0 && (s = m.fill(X[l], m.cells(l) * m.ncomp(), st)) != PPPM_OK) return s;
+        if ((s = m.smooth(l, X[l], B[l], bs, nu1, omega, beta, st)) != PPPM_OK) return s;
+        if ((s = m.residual_restrict(l, X[l], B[l], bs, beta, B[l + 1], st)) != PPPM_OK) return s;
+    }
+    {
+        T bs = (L - 1) == 0 ? bscale0 : (T)1;
+        if ((s = m.coarse(X[L - 1], B[L - 1], bs, consistent, st)) != PPPM_OK) return s;
+    }
+    for (int l = L - 2; l >= l0; l--) {
+        T beta = consistent ? (T)mg_beta(l) : (T)0;
+        T bs = l == 0 ? bscale0 : (T)1;
+        T betac = consistent ? (T)mg_beta(l + 1) : (T)0;
+        if ((s = m.prolong(l, X[l], X[l + 1], consistent ? PROLONG_TRILINEAR : PROLONG_CONSTANT, betac, st)) !=
+            PPPM_OK)
+            return s;
+        if ((s = m.smooth(l, X[l], B[l], bs, nu2, omega, beta, st)) != PPPM_OK) return s;
+    }
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent,
+                              hipStream_t st) {
+    std::vector<T*> X = x_, B = b_;
+    X[0] = x0;
+    B[0] = const_cast<T*>(b0);
+    return vcycle_from(*this, X, B, 0, bscale0, nu1, nu2, omega, consistent, st);
+}
+
+template <typename T>
+pppm_status MGHier<T>::ref_fmg(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, hipStream_t st) {
+    std::vector<T*> X = x_, B = b_;
+    X[0] = x0;
+    B[0] = const_cast<T*>(b0);
+    pppm_status s;
+    for (int l = 1; l < L_; l++)
+        if ((s = fill(X[l], cells(l) * ncomp_, st)) != PPPM_OK) return s;
+    for (int l = 0; l < L_ - 1; l++) {
+        T bs = l == 0 ? bscale0 : (T)1;
+        if ((s = residual_restrict(l, X[l], B[l], bs, (T)0, B[l + 1], st)) != PPPM_OK) return s;
+    }
+    if ((s = coarse(X[L_ - 1], B[L_ - 1], L_ == 1 ? bscale0 : (T)1, false, st)) != PPPM_OK) return s;
+    for (int l = L_ - 2; l >= 0; l--) {
+        if ((s = prolong(l, X[l], X[l + 1], PROLONG_CONSTANT, (T)0, st)) != PPPM_OK) return s;
+        if ((s = vcycle_from(*this, X, B, l, bscale0, nu1, nu2, omega, false, st)) != PPPM_OK) return s;
+    }
+    return vcycle_from(*this, X, B, 0, bscale0, nu1, nu2, omega, false, st);
+}
+
+template <typename T>
+pppm_status MGHier<T>::norms(const T* x0, const T* b0, T bscale0, double* rn, double* bn, hipStream_t st) {
+    const int nx = d_[0][0], ny = d_[0][1], nz = d_[0][2];
+    double* out = part_ + (size_t)kNormBlocks * 3 * 2;
+    hipLaunchKernelGGL(k_resnorm<T>, dim3(kNormBlocks, ncomp_), dim3(256), 0, st, x0, b0, bscale0, nx, ny, nz,
+                       cells(0), part_);
+    PPPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp_), dim3(256), 0, st, part_, kNormBlocks, out);
+    PPPM_LAUNCH_CHECK();
+    PPPM_HIP_TRY(hipMemcpyAsync(hnorm_, out, 2 * ncomp_ * sizeof(double), hipMemcpyDeviceToHost, st));
+    PPPM_HIP_TRY(hipStreamSynchronize(st));
+    for (int c = 0; c < ncomp_; c++) {
+        rn[c] = std::sqrt(hnorm_[2 * c]);
+        bn[c] = std::sqrt(hnorm_[2 * c + 1]);
+    }
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::solve(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, double tol, int max_cycles,
+                             int* cycles, double* relres, hipStream_t st) {
+    double rn[3], bn[3];
+    pppm_status s;
+    int cyc = 0;
+    auto worst = [&]() {
+        double w = 0;
+        for (int c = 0; c < ncomp_; c++) {
+            double r = bn[c] > 0 ? rn[c] / bn[c] : rn[c];
+            relres[c] = r;
+            if (r > w) w = r;
+        }
+        return w;
+    };
+    if ((s = norms(x0, b0, bscale0, rn, bn, st)) != PPPM_OK) return s;
+    double w = worst();
+    while (cyc < max_cycles && w > tol) {
+        if ((s = vcycle(x0, b0, bscale0, nu1, nu2, omega, true, st)) != PPPM_OK) return s;
+        cyc++;
+        if ((s = norms(x0, b0, bscale0, rn, bn, st)) != PPPM_OK) return s;
+        double prev = w;
+        w = worst();
+        // rounding floor of the working precision reached: stop
+        if (cyc >= 3 && w > 0.9 * prev) break;
+    }
+    *cycles = cyc;
+    return w <= tol ? PPPM_OK : PPPM_ERR_NOT_CONVERGED;
+}
+
+template class MGHier<float>;
+template class MGHier<double>;
+
+// ---------------------------------------------------------------------------
+// Single-array primitives behind the reference's mg_* entry points
+// (Multigrid.cu:46-60, 202-330).
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual_full(const T* __restrict__ x, const T* __restrict__ b,
+                                                       T* __restrict__ r, int nx, int ny, int nz) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    r[((size_t)k * ny + j) * nx + i] = residual_at(x, b, (T)1, i, j, k, nx, ny, nz, (T)0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict_full(T* __restrict__ bc, const T* __restrict__ rf, int cx, int cy,
+                                                       int cz) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= cx || j >= cy) return;
+    const size_t nx = 2 * (size_t)cx, ny = 2 * (size_t)cy;
+    const size_t a = ((size_t)(2 * k) * ny + 2 * j) * nx + 2 * i;
+    const size_t ay = a + nx, az = a + nx * ny, ayz = az + nx;
+    T r = 0;
+    r += rf[a];
+    r += rf[a + 1];
+    r += rf[ay];
+    r += rf[ay + 1];
+    r += rf[az];
+    r += rf[az + 1];
+    r += rf[ayz];
+    r += rf[ayz + 1];
+    bc[((size_t)k * cy + j) * cx + i] = r * (T)0.5;
+}
+
+template <typename T>
+pppm_status ref_rbgs(T* x, const T* b, int iters, int nx, int ny, int nz, T omega, hipStream_t st) {
+    dim3 blk(64, 4), grid(div_up(div_up(nx, 2), 64), div_up(ny, 4), nz);
+    const size_t G = (size_t)nx * ny * nz;
+    for (int it = 0; it < iters; it++)
+        for (int color = 1; color >= 0; color--) {
+            hipLaunchKernelGGL(k_rbgs<T>, grid, blk, 0, st, x, b, (T)1, nx, ny, nz, G, color, omega, (T)0);
+            PPPM_LAUNCH_CHECK();
+        }
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status ref_residual(const T* x, const T* b, T* r, int nx, int ny, int nz, hipStream_t st) {
+    hipLaunchKernelGGL(k_residual_full<T>, dim3(div_up(nx, 64), div_up(ny, 4), nz), dim3(64, 4), 0, st, x, b, r, nx,
+                       ny, nz);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status ref_restrict(T* bc, const T* rf, int cx, int cy, int cz, hipStream_t st) {
+    hipLaunchKernelGGL(k_restrict_full<T>, dim3(div_up(cx, 64), div_up(cy, 4), cz), dim3(64, 4), 0, st, bc, rf, cx,
+                       cy, cz);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status ref_prolong(T* xf, const T* xc, int fx, int fy, int fz, hipStream_t st) {
+    int cx = fx / 2 > 0 ? fx / 2 : 1, cy = fy / 2 > 0 ? fy / 2 : 1, cz = fz / 2 > 0 ? fz / 2 : 1;
+    hipLaunchKernelGGL(k_prolong<T>, dim3(div_up(fx, 64), div_up(fy, 4), fz), dim3(64, 4), 0, st, xf, xc, fx, fy, fz,
+                       cx, cy, cz, (size_t)fx * fy * fz, (size_t)cx * cy * cz, (int)PROLONG_CONSTANT, (T)0);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status ref_coarse(T* res, const T* b, int nx, int ny, int nz, hipStream_t st) {
+    if ((long)nx * ny * nz > 512) {
+        pppm_set_last_error("coarse solve needs <= 512 cells", __FILE__, __LINE__);
+        return PPPM_ERR_INVALID;
+    }
+    std::vector<double> t = coarse_tables(nx, ny, nz, 0.0);
+    double* d = nullptr;
+    PPPM_HIP_TRY(hipMalloc(&d, t.size() * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(512), 0, st, res, b, (T)1, nx, ny, nz, (size_t)nx * ny * nz, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    PPPM_HIP_TRY(e);
+    return PPPM_OK;
+}
+
+#define PPPM_INST_REF(T)                                                                                  \
+    template pppm_status ref_rbgs<T>(T*, const T*, int, int, int, int, T, hipStream_t);                    \
+    template pppm_status ref_residual<T>(const T*, const T*, T*, int, int, int, hipStream_t);              \
+    template pppm_status ref_restrict<T>(T*, const T*, int, int, int, hipStream_t);                        \
+    template pppm_status ref_prolong<T>(T*, const T*, int, int, int, hipStream_t);                         \
+    template pppm_status ref_coarse<T>(T*, const T*, int, int, int, hipStream_t);
+PPPM_INST_REF(float)
+PPPM_INST_REF(double)
+
+}  // namespace pppm

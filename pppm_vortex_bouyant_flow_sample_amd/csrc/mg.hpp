@@ -1,0 +1,90 @@
+// Geometric multigrid hierarchy for the cell-centred 7-point Poisson problem
+// sum_nb(x) - 6 x = b with zero ghost values (multigrid/multigrid/MultiGrid.h,
+// MultiGrid.cpp, MultiGrid_kernel.cu).  Several independent components (the
+// three stream-function components) are solved together: arrays are
+// component-major [ncomp][nx*ny*nz].
+#pragma once
+#include "pppm_internal.hpp"
+#include <vector>
+
+namespace pppm {
+
+enum { PROLONG_CONSTANT = 0, PROLONG_TRILINEAR = 1 };
+
+// Ghost factor of level l of the tolerance-driven cycle (l = 0: 0).
+double mg_beta(int level);
+
+template <typename T>
+class MGHier {
+   public:
+    MGHier() = default;
+    ~MGHier() { release(); }
+    MGHier(const MGHier&) = delete;
+    MGHier& operator=(const MGHier&) = delete;
+
+    pppm_status init(int nx, int ny, int nz, int ncomp);
+    void release();
+    int levels() const { return L_; }
+    int ncomp() const { return ncomp_; }
+    const int* dims(int l) const { return d_[l]; }
+    size_t cells(int l) const { return (size_t)d_[l][0] * d_[l][1] * d_[l][2]; }
+    size_t device_bytes() const { return bytes_; }
+
+    // One V(nu1,nu2) cycle from level l0 on (x0, b0 * bscale0) at level 0.
+    // consistent: trilinear prolongation and boundary-consistent coarse
+    // operators (tolerance-driven cycle); otherwise the reference pair
+    // (piecewise-constant prolongation, zero ghosts on every level).
+    pppm_status vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent, hipStream_t st);
+    // MultiGrid.cpp:202-250 (m_FullMultiGrid): one FMG pass plus one V-cycle.
+    pppm_status ref_fmg(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, hipStream_t st);
+    // ||b*bscale - A x||_2 and ||b*bscale||_2 per component (FP64 sums), synchronous.
+    pppm_status norms(const T* x0, const T* b0, T bscale0, double* rn, double* bn, hipStream_t st);
+    // Repeated V-cycles until max relative residual <= tol or max_cycles.
+    pppm_status solve(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, double tol, int max_cycles,
+                      int* cycles, double* relres /*[ncomp]*/, hipStream_t st);
+
+    // raw pieces (also used by the reference-named entry points)
+    pppm_status smooth(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st);
+    pppm_status residual_restrict(int l, const T* x, const T* b, T bscale, T beta, T* bnext, hipStream_t st);
+    pppm_status prolong(int l, T* x, const T* xc, int mode, T betac, hipStream_t st);
+    pppm_status coarse(T* x, const T* b, T bscale, bool consistent, hipStream_t st);
+    pppm_status fill(T* p, size_t n, hipStream_t st);
+    double* partials() { return part_; }
+    double* host_norms() { return hnorm_; }
+    static constexpr int kNormBlocks = 512;
+
+   private:
+    int L_ = 0, ncomp_ = 1;
+    int d_[16][3];
+    std::vector<T*> x_, b_;       // levels >= 1
+    double* tab_cons_ = nullptr;  // coarse eigenbasis, consistent ghosts
+    double* tab_ref_ = nullptr;   // coarse eigenbasis, zero ghosts
+    double* part_ = nullptr;      // norm partial sums
+    double* hnorm_ = nullptr;     // pinned host copy of the reduced norms
+    size_t bytes_ = 0;
+};
+
+// FP64 defect-correction step of the mixed-precision solve:
+// x_out = x_in + e (x_in, e may be null = 0), r32 = (float)(b - A x_out),
+// block partial sums of r^2 and b^2 into partials; reduced per component into
+// out[2*c] = ||r||^2, out[2*c+1] = ||b||^2 (device, FP64).
+pppm_status launch_dc_step(const double* x_in, const float* e, const double* b64, double* x_out, float* r32, int nx,
+                           int ny, int nz, int ncomp, double* partials, double* out, hipStream_t st);
+
+// Coarse eigenbasis tables: E_x, E_y, E_z (row k = eigenvector k), lam_x, lam_y,
+// lam_z of the 1-D second difference with out-of-grid neighbour = beta * end value.
+std::vector<double> coarse_tables(int nx, int ny, int nz, double beta);
+
+// Single-array primitives of the reference's mg_* entry points.
+template <typename T>
+pppm_status ref_rbgs(T* x, const T* b, int iters, int nx, int ny, int nz, T omega, hipStream_t st);
+template <typename T>
+pppm_status ref_residual(const T* x, const T* b, T* r, int nx, int ny, int nz, hipStream_t st);
+template <typename T>
+pppm_status ref_restrict(T* bc, const T* rf, int cx, int cy, int cz, hipStream_t st);
+template <typename T>
+pppm_status ref_prolong(T* xf, const T* xc, int fx, int fy, int fz, hipStream_t st);
+template <typename T>
+pppm_status ref_coarse(T* res, const T* b, int nx, int ny, int nz, hipStream_t st);
+
+}  // namespace pppm

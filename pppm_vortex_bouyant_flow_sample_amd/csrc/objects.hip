@@ -1,0 +1,281 @@
+// C ABI of the standalone spatial hash (ParticleManager, particle_hash/
+// particle_hash/SpaceHashClass.h:10-84, SpaceHashClassImp.cpp:18-65) and of the
+// standalone multigrid solver (gf::MultiGrid / MultiGridSolver3DPeriod(_double),
+// multigrid/multigrid/MultiGrid.h:29-209), plus primitives on caller buffers.
+#include "mg.hpp"
+#include <cstring>
+
+using namespace pppm;
+
+#define FAIL(code, msg)                                  \
+    do {                                                 \
+        pppm_set_last_error((msg), __FILE__, __LINE__);  \
+        return (code);                                   \
+    } while (0)
+#define TRY(expr)                     \
+    do {                              \
+        pppm_status _s = (expr);      \
+        if (_s != PPPM_OK) return _s; \
+    } while (0)
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// ---------------------------------------------------------------------------
+struct pppm_hash {
+    uint32_t cap = 0, n = 0;
+    HashGeom hg{};
+    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    uint32_t *ks = nullptr, *vs = nullptr;
+    uint32_t *start = nullptr, *end = nullptr;
+    uint64_t ncells_alloc = 0;
+    void* scratch = nullptr;
+    void release() {
+        for (uint32_t** p : {&k0, &v0, &k1, &v1, &start, &end})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        if (scratch) (void)hipFree(scratch);
+        scratch = nullptr;
+        cap = 0;
+        ncells_alloc = 0;
+    }
+};
+
+static pppm_status hash_alloc(pppm_hash* hs, uint32_t cap, uint64_t ncells) {
+    if (cap > hs->cap || !hs->k0) {
+        for (uint32_t** p : {&hs->k0, &hs->v0, &hs->k1, &hs->v1})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        if (hs->scratch) (void)hipFree(hs->scratch);
+        hs->scratch = nullptr;
+        uint32_t c = cap < 1024 ? 1024 : cap;
+        for (uint32_t** p : {&hs->k0, &hs->v0, &hs->k1, &hs->v1})
+            if (hipMalloc((void**)p, (size_t)c * 4) != hipSuccess) FAIL(PPPM_ERR_NOMEM, "hash alloc");
+        if (hipMalloc(&hs->scratch, sort_scratch_bytes(c)) != hipSuccess) FAIL(PPPM_ERR_NOMEM, "hash alloc");
+        hs->cap = c;
+    }
+    if (ncells > hs->ncells_alloc || !hs->start) {
+        if (hs->start) (void)hipFree(hs->start);
+        if (hs->end) (void)hipFree(hs->end);
+        hs->start = hs->end = nullptr;
+        if (hipMalloc((void**)&hs->start, ncells * 4) != hipSuccess ||
+            hipMalloc((void**)&hs->end, ncells * 4) != hipSuccess)
+            FAIL(PPPM_ERR_NOMEM, "hash alloc");
+        hs->ncells_alloc = ncells;
+    }
+    return PPPM_OK;
+}
+
+static pppm_status check_geom(uint32_t g, float h) {
+    if (g < 1 || g > 1024) FAIL(PPPM_ERR_INVALID, "hash grid must be 1..1024 cells per axis");
+    if (!(h > 0)) FAIL(PPPM_ERR_INVALID, "hash cell size must be positive");
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_hash_create(uint32_t capacity, uint32_t g, float h, const float origin[3],
+                                        pppm_hash** out) {
+    if (!out || !origin) FAIL(PPPM_ERR_INVALID, "null arg");
+    *out = nullptr;
+    TRY(check_geom(g, h));
+    pppm_hash* hs = new pppm_hash();
+    hs->hg.g = g;
+    hs->hg.hf = h;
+    for (int i = 0; i < 3; i++) hs->hg.of[i] = origin[i];
+    pppm_status s = hash_alloc(hs, capacity, (uint64_t)g * g * g);
+    if (s != PPPM_OK) {
+        hs->release();
+        delete hs;
+        return s;
+    }
+    *out = hs;
+    return PPPM_OK;
+}
+
+extern "C" void pppm_hash_destroy(pppm_hash* hs) {
+    if (!hs) return;
+    (void)hipDeviceSynchronize();
+    hs->release();
+    delete hs;
+}
+
+extern "C" pppm_status pppm_hash_set_grid(pppm_hash* hs, uint32_t g, float h, const float origin[3]) {
+    if (!hs || !origin) FAIL(PPPM_ERR_INVALID, "null arg");
+    TRY(check_geom(g, h));
+    hs->hg.g = g;
+    hs->hg.hf = h;
+    for (int i = 0; i < 3; i++) hs->hg.of[i] = origin[i];
+    return hash_alloc(hs, hs->cap, (uint64_t)g * g * g);
+}
+
+extern "C" pppm_status pppm_hash_build(pppm_hash* hs, const float* pos4, uint32_t n, void* stream) {
+    if (!hs) FAIL(PPPM_ERR_INVALID, "null hash");
+    if (n > 0 && !pos4) FAIL(PPPM_ERR_INVALID, "null positions");
+    // the reference divides by zero for n == 0 (SpaceHashFunctional.cu:41-51);
+    // here an empty build gives all-empty tables
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t ncells = (uint64_t)hs->hg.g * hs->hg.g * hs->hg.g;
+    TRY(hash_alloc(hs, n, ncells));
+    hs->n = n;
+    TRY(launch_calc_keys((const float4*)pos4, n, hs->hg, hs->k0, nullptr, st));
+    TRY(radix_sort_pairs(hs->k0, hs->v0, hs->k1, hs->v1, n, key_bits_for(ncells), true, hs->scratch, &hs->ks, &hs->vs,
+                         st));
+    TRY(launch_cell_tables(hs->ks, n, (uint32_t)ncells, hs->start, hs->end, st));
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_hash_tables(pppm_hash* hs, const uint32_t** key_sorted, const uint32_t** index_sorted,
+                                        const uint32_t** cell_start, const uint32_t** cell_end) {
+    if (!hs) FAIL(PPPM_ERR_INVALID, "null hash");
+    if (key_sorted) *key_sorted = hs->ks ? hs->ks : hs->k0;
+    if (index_sorted) *index_sorted = hs->vs ? hs->vs : hs->v0;
+    if (cell_start) *cell_start = hs->start;
+    if (cell_end) *cell_end = hs->end;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_hash_reorder(pppm_hash* hs, uint32_t n, const void* in, void* out, uint32_t stride,
+                                         int32_t dtype, void* stream) {
+    if (!hs || (n && (!in || !out))) FAIL(PPPM_ERR_INVALID, "null arg");
+    if (n > hs->n) FAIL(PPPM_ERR_INVALID, "n exceeds the hashed count");
+    return launch_gather(hs->vs, in, out, stride, dtype, n, (hipStream_t)stream);
+}
+
+extern "C" pppm_status pppm_hash_deorder(pppm_hash* hs, uint32_t n, const void* in, void* out, uint32_t stride,
+                                         int32_t dtype, void* stream) {
+    if (!hs || (n && (!in || !out))) FAIL(PPPM_ERR_INVALID, "null arg");
+    if (n > hs->n) FAIL(PPPM_ERR_INVALID, "n exceeds the hashed count");
+    return launch_scatter(hs->vs, in, out, stride, dtype, n, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" pppm_status pppm_calc_keys(const float* pos4, uint32_t n, uint32_t g, float h, const float origin[3],
+                                      uint32_t* keys, uint32_t* idx, void* stream) {
+    if (n && (!pos4 || !keys)) FAIL(PPPM_ERR_INVALID, "null arg");
+    TRY(check_geom(g, h));
+    HashGeom hg{g, h, {origin[0], origin[1], origin[2]}};
+    return launch_calc_keys((const float4*)pos4, n, hg, keys, idx, (hipStream_t)stream);
+}
+
+extern "C" size_t pppm_sort_scratch_bytes(uint32_t n) {
+    uint32_t c = n ? n : 1;
+    return 2 * align256((size_t)c * 4) + sort_scratch_bytes(c);
+}
+
+extern "C" pppm_status pppm_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* scratch,
+                                       void* stream) {
+    if (n && (!keys || !vals || !scratch)) FAIL(PPPM_ERR_INVALID, "null arg");
+    if (key_bits == 0 || key_bits > 32) FAIL(PPPM_ERR_INVALID, "key_bits");
+    if (n == 0) return PPPM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    uint8_t* sp = (uint8_t*)scratch;
+    uint32_t* k1 = (uint32_t*)sp;
+    uint32_t* v1 = (uint32_t*)(sp + align256((size_t)n * 4));
+    void* rs = sp + 2 * align256((size_t)n * 4);
+    uint32_t *ko, *vo;
+    TRY(radix_sort_pairs(keys, vals, k1, v1, n, key_bits, false, rs, &ko, &vo, st));
+    if (ko != keys) {
+        PPPM_HIP_TRY(hipMemcpyAsync(keys, ko, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        PPPM_HIP_TRY(hipMemcpyAsync(vals, vo, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_cell_tables(const uint32_t* keys_sorted, uint32_t n, uint32_t ncells, uint32_t* start,
+                                        uint32_t* end, void* stream) {
+    if ((n && !keys_sorted) || !start || !end) FAIL(PPPM_ERR_INVALID, "null arg");
+    return launch_cell_tables(keys_sorted, n, ncells, start, end, (hipStream_t)stream);
+}
+
+extern "C" pppm_status pppm_gather(const uint32_t* idx, const void* in, void* out, uint32_t stride, int32_t dtype,
+                                   uint32_t n, void* stream) {
+    if (n && (!idx || !in || !out)) FAIL(PPPM_ERR_INVALID, "null arg");
+    return launch_gather(idx, in, out, stride, dtype, n, (hipStream_t)stream);
+}
+
+extern "C" pppm_status pppm_scatter(const uint32_t* idx, const void* in, void* out, uint32_t stride, int32_t dtype,
+                                    uint32_t n, void* stream) {
+    if (n && (!idx || !in || !out)) FAIL(PPPM_ERR_INVALID, "null arg");
+    return launch_scatter(idx, in, out, stride, dtype, n, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+struct pppm_mg {
+    int dtype = PPPM_F64;
+    MGHier<float> f;
+    MGHier<double> d;
+};
+
+extern "C" pppm_status pppm_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, pppm_mg** out) {
+    if (!out) FAIL(PPPM_ERR_INVALID, "null out");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) FAIL(PPPM_ERR_INVALID, "grid dims");
+    if (dtype != PPPM_F32 && dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "dtype");
+    pppm_mg* m = new pppm_mg();
+    m->dtype = dtype;
+    pppm_status s = dtype == PPPM_F32 ? m->f.init(nx, ny, nz, 1) : m->d.init(nx, ny, nz, 1);
+    if (s != PPPM_OK) {
+        delete m;
+        return s;
+    }
+    *out = m;
+    return PPPM_OK;
+}
+
+extern "C" void pppm_mg_destroy(pppm_mg* m) {
+    if (!m) return;
+    (void)hipDeviceSynchronize();
+    delete m;
+}
+
+extern "C" pppm_status pppm_mg_levels(const pppm_mg* m, int32_t* levels, int32_t dims_out[][3], int32_t max_levels) {
+    if (!m || !levels) FAIL(PPPM_ERR_INVALID, "null arg");
+    int L = m->dtype == PPPM_F32 ? m->f.levels() : m->d.levels();
+    *levels = L;
+    for (int l = 0; l < L && l < max_levels && dims_out; l++) {
+        const int* d = m->dtype == PPPM_F32 ? m->f.dims(l) : m->d.dims(l);
+        dims_out[l][0] = d[0];
+        dims_out[l][1] = d[1];
+        dims_out[l][2] = d[2];
+    }
+    return PPPM_OK;
+}
+
+template <typename T>
+static pppm_status mg_solve_t(MGHier<T>& h, T* x, const T* b, int32_t schedule, int32_t nu1, int32_t nu2,
+                              double omega, double tol, int32_t max_cycles, int32_t* cycles_out, double* relres_out,
+                              hipStream_t st) {
+    double rel[3] = {0, 0, 0};
+    int cyc = 0;
+    pppm_status s;
+    if (schedule == PPPM_MG_REF_FMG) {
+        TRY(h.ref_fmg(x, b, (T)1, nu1, nu2, (T)omega, st));
+        double rn, bn;
+        TRY(h.norms(x, b, (T)1, &rn, &bn, st));
+        rel[0] = bn > 0 ? rn / bn : rn;
+        cyc = 1;
+        s = PPPM_OK;
+    } else {
+        s = h.solve(x, b, (T)1, nu1, nu2, (T)omega, tol, max_cycles, &cyc, rel, st);
+        if (s != PPPM_OK && s != PPPM_ERR_NOT_CONVERGED) return s;
+    }
+    if (cycles_out) *cycles_out = cyc;
+    if (relres_out) *relres_out = rel[0];
+    return s;
+}
+
+extern "C" pppm_status pppm_mg_solve(pppm_mg* m, void* x, const void* b, int32_t schedule, int32_t nu1, int32_t nu2,
+                                     double omega, double tol, int32_t max_cycles, int32_t* cycles_out,
+                                     double* relres_out, void* stream) {
+    if (!m || !x || !b) FAIL(PPPM_ERR_INVALID, "null arg");
+    if (schedule != PPPM_MG_VCYCLE && schedule != PPPM_MG_REF_FMG) FAIL(PPPM_ERR_INVALID, "schedule");
+    if (nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || max_cycles < 1) FAIL(PPPM_ERR_INVALID, "cycle parameters");
+    hipStream_t st = (hipStream_t)stream;
+    if (m->dtype == PPPM_F32)
+        return mg_solve_t(m->f, (float*)x, (const float*)b, schedule, nu1, nu2, omega, tol, max_cycles, cycles_out,
+                          relres_out, st);
+    return mg_solve_t(m->d, (double*)x, (const double*)b, schedule, nu1, nu2, omega, tol, max_cycles, cycles_out,
+                      relres_out, st);
+}

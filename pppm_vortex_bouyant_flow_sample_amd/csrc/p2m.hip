@@ -1,0 +1,109 @@
+// P2M: nearest-grid-point spreading of the sorted vortex strengths onto the
+// cell-centred grid (GPU_kernel.cu:218-294 ParticleToMesh + 524-543 ComputeRHS).
+// Each vortex carries its deposit cell in spos.w, so a cell reduces the
+// contiguous sorted segment of its own hash cell; no atomics.  Vortices whose
+// deposit cell differs from their hash cell ("strays", only possible when the
+// FP32 hash and the FP64 deposit rule round differently at a cell face) are
+// added by a second gather over the +-2 hash-cell neighbourhood, the range the
+// reference gathers from.
+#include "pppm_internal.hpp"
+
+namespace pppm {
+
+template <typename Acc>
+__global__ void __launch_bounds__(256) k_p2m(const uint32_t* __restrict__ off, const float4* __restrict__ spos,
+                                             const float4* __restrict__ somg, uint32_t G, double invh3,
+                                             float* __restrict__ wg, double* __restrict__ b64, double bscale64) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G) return;
+    const uint32_t n = off[G];
+    uint32_t s = off[c], e = min(off[c + 1], n);
+    Acc ax = 0, ay = 0, az = 0;
+    for (uint32_t p = s; p < e; p++) {
+        if (__float_as_uint(spos[p].w) != c) continue;
+        float4 o = somg[p];
+        ax += (Acc)o.x;
+        ay += (Acc)o.y;
+        az += (Acc)o.z;
+    }
+    wg[c] = (float)(invh3 * (double)ax);
+    wg[(size_t)G + c] = (float)(invh3 * (double)ay);
+    wg[2 * (size_t)G + c] = (float)(invh3 * (double)az);
+    if (b64) {
+        b64[c] = bscale64 * (invh3 * (double)ax);
+        b64[(size_t)G + c] = bscale64 * (invh3 * (double)ay);
+        b64[2 * (size_t)G + c] = bscale64 * (invh3 * (double)az);
+    }
+}
+
+template <typename Acc>
+__global__ void __launch_bounds__(256) k_p2m_strays(const uint32_t* __restrict__ off, const float4* __restrict__ spos,
+                                                    const float4* __restrict__ somg, uint32_t g, double invh3,
+                                                    float* __restrict__ wg, double* __restrict__ b64,
+                                                    double bscale64) {
+    const uint32_t G = g * g * g;
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G) return;
+    const int i = (int)(c % g), j = (int)((c / g) % g), k = (int)(c / (g * g));
+    Acc ax = 0, ay = 0, az = 0;
+    bool any = false;
+    for (int kk = k - 2; kk <= k + 2; kk++) {
+        if (kk < 0 || kk >= (int)g) continue;
+        for (int jj = j - 2; jj <= j + 2; jj++) {
+            if (jj < 0 || jj >= (int)g) continue;
+            for (int ii = i - 2; ii <= i + 2; ii++) {
+                if (ii < 0 || ii >= (int)g) continue;
+                uint32_t nc = ((uint32_t)kk * g + (uint32_t)jj) * g + (uint32_t)ii;
+                if (nc == c) continue;
+                const uint32_t pe = min(off[nc + 1], off[G]);
+                for (uint32_t p = off[nc]; p < pe; p++) {
+                    if (__float_as_uint(spos[p].w) != c) continue;
+                    float4 o = somg[p];
+                    ax += (Acc)o.x;
+                    ay += (Acc)o.y;
+                    az += (Acc)o.z;
+                    any = true;
+                }
+            }
+        }
+    }
+    if (!any) return;
+    wg[c] += (float)(invh3 * (double)ax);
+    wg[(size_t)G + c] += (float)(invh3 * (double)ay);
+    wg[2 * (size_t)G + c] += (float)(invh3 * (double)az);
+    if (b64) {
+        b64[c] += bscale64 * (invh3 * (double)ax);
+        b64[(size_t)G + c] += bscale64 * (invh3 * (double)ay);
+        b64[2 * (size_t)G + c] += bscale64 * (invh3 * (double)az);
+    }
+}
+
+pppm_status launch_p2m(const uint32_t* off, const float4* spos, const float4* somg, uint32_t g, double invh,
+                       float* wgrid, double* b64, double bscale64, hipStream_t st) {
+    const uint32_t G = g * g * g;
+    const double invh3 = invh * invh * invh;
+    if (b64)
+        hipLaunchKernelGGL(k_p2m<double>, dim3(div_up(G, 256)), dim3(256), 0, st, off, spos, somg, G, invh3, wgrid,
+                           b64, bscale64);
+    else
+        hipLaunchKernelGGL(k_p2m<float>, dim3(div_up(G, 256)), dim3(256), 0, st, off, spos, somg, G, invh3, wgrid,
+                           b64, bscale64);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+pppm_status launch_p2m_strays(const uint32_t* off, const float4* spos, const float4* somg, uint32_t g, double invh,
+                              float* wgrid, double* b64, double bscale64, hipStream_t st) {
+    const uint32_t G = g * g * g;
+    const double invh3 = invh * invh * invh;
+    if (b64)
+        hipLaunchKernelGGL(k_p2m_strays<double>, dim3(div_up(G, 256)), dim3(256), 0, st, off, spos, somg, g, invh3,
+                           wgrid, b64, bscale64);
+    else
+        hipLaunchKernelGGL(k_p2m_strays<float>, dim3(div_up(G, 256)), dim3(256), 0, st, off, spos, somg, g, invh3,
+                           wgrid, b64, bscale64);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+}  // namespace pppm

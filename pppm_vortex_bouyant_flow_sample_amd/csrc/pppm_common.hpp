@@ -1,0 +1,76 @@
+// Shared device/host helpers for the gfx950 PPPM library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+#define PPPM_WAVE 64
+
+#define PPPM_HIP_TRY(expr)                                                   \
+    do {                                                                     \
+        hipError_t _e = (expr);                                              \
+        if (_e != hipSuccess) {                                              \
+            pppm_set_last_error(hipGetErrorString(_e), __FILE__, __LINE__);  \
+            return PPPM_ERR_HIP;                                             \
+        }                                                                    \
+    } while (0)
+
+#define PPPM_LAUNCH_CHECK() PPPM_HIP_TRY(hipGetLastError())
+
+extern "C" void pppm_set_last_error(const char* msg, const char* file, int line);
+
+namespace pppm {
+
+constexpr uint32_t kEmptyCell = 0xFFFFFFFFu;
+// 4 pi as the reference writes it (GPU_kernel.cu:14, 2395)
+constexpr double kRefFourPi = 12.5663706144;
+// diagonal stream-function coefficient (GPU_kernel.cu:1989)
+constexpr double kRefDiagPsi = 0.250580605913965;
+
+__host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t div_up64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// float -> int as the GPU hardware converts (round toward zero, saturating, NaN -> 0)
+__device__ inline int f2i_sat(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483648.0f) return 2147483647;
+    if (f < -2147483648.0f) return (-2147483647 - 1);
+    return (int)f;
+}
+__device__ inline int d2i_sat(double f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483648.0) return 2147483647;
+    if (f < -2147483648.0) return (-2147483647 - 1);
+    return (int)f;
+}
+
+// Hash cell of a position (particle_hash_kernel.cu:32-48): FP32 subtract and
+// correctly rounded FP32 divide, floor, saturating convert, unsigned wrap.
+__device__ inline uint32_t hash_cell_key(float px, float py, float pz, uint32_t g, float hf,
+                                         float ox, float oy, float oz) {
+    float qx = __fdiv_rn(__fsub_rn(px, ox), hf);
+    float qy = __fdiv_rn(__fsub_rn(py, oy), hf);
+    float qz = __fdiv_rn(__fsub_rn(pz, oz), hf);
+    uint32_t ux = (uint32_t)f2i_sat(floorf(qx)) % g;
+    uint32_t uy = (uint32_t)f2i_sat(floorf(qy)) % g;
+    uint32_t uz = (uint32_t)f2i_sat(floorf(qz)) % g;
+    return (uz * g + uy) * g + ux;
+}
+
+// Position relative to the double origin, rounded to float
+// (GPU_kernel.cu:252-255: float4 ppos; ppos.x -= origin.x).
+__device__ inline float shift_coord(float p, double o) { return __double2float_rn((double)p - o); }
+
+// Cell index from a shifted float coordinate in double (GPU_kernel.cu:2303, 2910).
+__device__ inline int cell_of(float q, double invh) { return d2i_sat(floor((double)q * invh)); }
+
+__device__ inline float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace pppm

@@ -1,0 +1,90 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+#include "pppm_common.hpp"
+#include "../../include/pppm.h"
+
+namespace pppm {
+
+// ---- scan ----------------------------------------------------------------
+size_t scan_scratch_words(uint64_t n);
+pppm_status exclusive_scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* scratch, hipStream_t st);
+pppm_status cell_offsets_from_tables(const uint32_t* start, const uint32_t* end, uint32_t ncells, uint32_t* off,
+                                     uint32_t* scratch, hipStream_t st);
+
+// ---- hash / sort ---------------------------------------------------------
+struct HashGeom {
+    uint32_t g;
+    float hf;
+    float of[3];
+};
+size_t sort_scratch_bytes(uint32_t n);
+uint32_t key_bits_for(uint64_t ncells);
+pppm_status launch_calc_keys(const float4* pos, uint32_t n, const HashGeom& hg, uint32_t* keys, uint32_t* idx,
+                             hipStream_t st);
+// Sorts (keys, vals) by key, stable; on return *keys_out/*vals_out point at
+// whichever buffer pair holds the result.  vals_identity: the input values
+// are 0..n-1 and vals is not read.
+pppm_status radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+                             uint32_t key_bits, bool vals_identity, void* scratch, uint32_t** keys_out,
+                             uint32_t** vals_out, hipStream_t st);
+pppm_status launch_fill_u32(uint32_t* a, uint64_t n, uint32_t v, hipStream_t st);
+pppm_status launch_cell_tables(const uint32_t* keys_sorted, uint32_t n, uint32_t ncells, uint32_t* start,
+                               uint32_t* end, hipStream_t st);
+pppm_status launch_gather(const uint32_t* idx, const void* in, void* out, uint32_t stride, int dtype, uint32_t n,
+                          hipStream_t st);
+pppm_status launch_scatter(const uint32_t* idx, const void* in, void* out, uint32_t stride, int dtype, uint32_t n,
+                           hipStream_t st);
+
+// Geometry of the PPPM path as the reference uses it: hash in float,
+// P2M/M2P/near field with the double origin and (double)(float)h.
+struct PathGeom {
+    HashGeom hash;
+    double origin[3];
+    double h;      // (double)(float)h
+    double invh;   // 1 / h
+};
+
+// Sorted vortex gather: spos = origin-shifted float coords + deposit-cell key
+// in .w (as bits; 0xFFFFFFFF outside the grid); somg = strengths (x,y,z,0).
+pppm_status launch_reorder_vortices(const uint32_t* idx, const uint32_t* keys_sorted, const float4* pos,
+                                    const void* wx, const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
+                                    uint32_t n, const PathGeom& pg, float4* spos, float4* somg, uint32_t* strays,
+                                    hipStream_t st);
+// Sorted evaluation points: shifted coords, optional other end (shifted).
+pppm_status launch_reorder_eval(const uint32_t* idx, const float4* pos, const float4* other, uint32_t m,
+                                const PathGeom& pg, float4* epos, float4* eoth, hipStream_t st);
+
+// ---- P2M -----------------------------------------------------------------
+pppm_status launch_p2m(const uint32_t* off, const float4* spos, const float4* somg, uint32_t g, double invh,
+                       float* wgrid /*[3][G]*/, double* b64 /*[3][G] or null*/, double bscale64, hipStream_t st);
+pppm_status launch_p2m_strays(const uint32_t* off, const float4* spos, const float4* somg, uint32_t g, double invh,
+                              float* wgrid, double* b64, double bscale64, hipStream_t st);
+
+// ---- far field -----------------------------------------------------------
+pppm_status launch_curl_far(const void* psi /*[3][G]*/, int psi_dtype, const float* wgrid, float* far, uint32_t g,
+                            int K, double h, hipStream_t st);
+
+// ---- M2P + near field ------------------------------------------------------
+struct NearArgs {
+    const float4* epos;      // sorted shifted eval coords
+    const float4* eoth;      // sorted shifted other ends or null
+    const uint32_t* eidx;    // eval sorted -> caller index
+    uint32_t m;
+    const uint32_t* off;     // vortex cell offsets [G+1]
+    const float4* spos;      // sorted shifted vortex coords
+    const float4* somg;      // sorted vortex strengths
+    const float* far;        // [3][G]
+    uint32_t g;
+    int K;
+    int kernel;
+    double invh;
+    double core_a;
+    void* ux;
+    void* uy;
+    void* uz;
+    uint32_t u_stride;
+    int u_dtype;
+};
+pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st);
+
+}  // namespace pppm

@@ -1,0 +1,212 @@
+// The reference's extern "C" entry points (include/pppm_ref.h) on top of this
+// library's kernels.  Like the reference they use one process-wide HashParams
+// (particle_hash_kernel.cu:27 keeps it in __constant__ memory) and the legacy
+// default stream; errors are recorded for pppm_last_error() instead of exiting.
+#include "mg.hpp"
+#include "../../include/pppm_ref.h"
+#include <cstring>
+#include <mutex>
+
+using namespace pppm;
+
+namespace {
+
+HashParams g_params;  // SpaceHashFunctional.cu:34-38 copies it to the device
+bool g_params_set = false;
+int g_coarse_dims[3] = {0, 0, 0};
+
+// grow-only scratch of sortParticles (Thrust allocates its own in the reference)
+std::mutex g_sort_mu;
+void* g_sort_scratch = nullptr;
+size_t g_sort_bytes = 0;
+
+void report(pppm_status s, const char* what) {
+    if (s != PPPM_OK && pppm_last_error()[0] == 0) pppm_set_last_error(what, __FILE__, __LINE__);
+}
+
+// particle_hash_kernel.cu:32-70 with per-axis grid size and cell size
+__global__ void __launch_bounds__(256) k_calc_hash_general(uint32_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                           const float4* __restrict__ pos, uint32_t n, uint32_t gx,
+                                                           uint32_t gy, uint32_t gz, float ox, float oy, float oz,
+                                                           float hx, float hy, float hz) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i];
+    uint32_t ux = (uint32_t)f2i_sat(floorf(__fdiv_rn(__fsub_rn(p.x, ox), hx))) % gx;
+    uint32_t uy = (uint32_t)f2i_sat(floorf(__fdiv_rn(__fsub_rn(p.y, oy), hy))) % gy;
+    uint32_t uz = (uint32_t)f2i_sat(floorf(__fdiv_rn(__fsub_rn(p.z, oz), hz))) % gz;
+    keys[i] = (uz * gy + uy) * gx + ux;
+    idx[i] = i;
+}
+
+}  // namespace
+
+extern "C" void setParameters(HashParams* hostParams) {
+    if (!hostParams) return;
+    g_params = *hostParams;
+    g_params_set = true;
+}
+
+extern "C" void calcHash(uint32_t* gridParticleHash, uint32_t* gridParticleIndex, float* pos, int numParticles) {
+    if (numParticles <= 0) return;
+    if (!g_params_set || !gridParticleHash || !gridParticleIndex || !pos) {
+        pppm_set_last_error("calcHash: setParameters first / null pointer", __FILE__, __LINE__);
+        return;
+    }
+    const HashParams& p = g_params;
+    if (!p.gridSize[0] || !p.gridSize[1] || !p.gridSize[2]) {
+        pppm_set_last_error("calcHash: empty grid", __FILE__, __LINE__);
+        return;
+    }
+    uint32_t n = (uint32_t)numParticles;
+    hipLaunchKernelGGL(k_calc_hash_general, dim3(div_up(n, 256)), dim3(256), 0, 0, gridParticleHash, gridParticleIndex,
+                       (const float4*)pos, n, p.gridSize[0], p.gridSize[1], p.gridSize[2], p.worldOrigin[0],
+                       p.worldOrigin[1], p.worldOrigin[2], p.cellSize[0], p.cellSize[1], p.cellSize[2]);
+    if (hipGetLastError() != hipSuccess) pppm_set_last_error("calcHash launch failed", __FILE__, __LINE__);
+}
+
+extern "C" void FindCellStart(uint32_t* cellStart, uint32_t* cellEnd, uint32_t* gridParticleHash,
+                              uint32_t* gridParticleIndex, uint32_t numParticles, uint32_t numCells) {
+    (void)gridParticleIndex;
+    report(launch_cell_tables(gridParticleHash, numParticles, numCells, cellStart, cellEnd, 0), "FindCellStart");
+}
+
+extern "C" void reorder(uint32_t* indexTable, void* data, uint32_t stride, void* out_data, uint32_t precision,
+                        uint32_t num_data) {
+    report(launch_gather(indexTable, data, out_data, stride, (int)precision, num_data, 0), "reorder");
+}
+
+extern "C" void deorder(uint32_t* indexTable, void* data, uint32_t stride, void* out_data, uint32_t precision,
+                        uint32_t num_data) {
+    report(launch_scatter(indexTable, data, out_data, stride, (int)precision, num_data, 0), "deorder");
+}
+
+extern "C" void sortParticles(uint32_t* dGridParticleHash, uint32_t* dGridParticleIndex, uint32_t numParticles) {
+    if (numParticles == 0) return;
+    std::lock_guard<std::mutex> lk(g_sort_mu);
+    size_t need = pppm_sort_scratch_bytes(numParticles);
+    if (need > g_sort_bytes) {
+        if (g_sort_scratch) (void)hipFree(g_sort_scratch);
+        g_sort_scratch = nullptr;
+        g_sort_bytes = 0;
+        if (hipMalloc(&g_sort_scratch, need) != hipSuccess) {
+            pppm_set_last_error("sortParticles: scratch allocation failed", __FILE__, __LINE__);
+            return;
+        }
+        g_sort_bytes = need;
+    }
+    uint32_t bits = 32;
+    if (g_params_set && g_params.numCells) bits = key_bits_for(g_params.numCells);
+    report(pppm_sort_pairs(dGridParticleHash, dGridParticleIndex, numParticles, bits, g_sort_scratch, nullptr),
+           "sortParticles");
+}
+
+// ---------------------------------------------------------------------------
+// multigrid (MultiGridcu.h); omega = 1.43f as MultiGrid_kernel.cu:36
+static const float kRefOmega = 1.43f;
+
+template <typename T>
+static void rbgs(T* x, T* b, T* res, int t, int nx, int ny, int nz) {
+    if (res != x && res && x)
+        (void)hipMemcpyAsync(res, x, (size_t)nx * ny * nz * sizeof(T), hipMemcpyDeviceToDevice, 0);
+    report(ref_rbgs<T>(res ? res : x, b, t, nx, ny, nz, (T)kRefOmega, 0), "mg_RBGS_periodic3D");
+}
+
+extern "C" void mg_InitIdxBuffer(int dimx, int dimy, int dimz) {
+    g_coarse_dims[0] = dimx;
+    g_coarse_dims[1] = dimy;
+    g_coarse_dims[2] = dimz;
+}
+extern "C" void mg_zerofy(void* data, size_t size) {
+    if (data && size && hipMemsetAsync(data, 0, size, 0) != hipSuccess)
+        pppm_set_last_error("mg_zerofy failed", __FILE__, __LINE__);
+}
+extern "C" void mg_exact_periodic3D(float* x, float* b, float* res, int dimx, int dimy, int dimz) {
+    (void)x;
+    report(ref_coarse<float>(res, b, dimx, dimy, dimz, 0), "mg_exact_periodic3D");
+}
+extern "C" void mg_exact_periodic3D2(double* x, double* b, double* res, int dimx, int dimy, int dimz) {
+    (void)x;
+    report(ref_coarse<double>(res, b, dimx, dimy, dimz, 0), "mg_exact_periodic3D2");
+}
+extern "C" void mg_RBGS_periodic3D(float* x, float* b, float* res, int t, int dimx, int dimy, int dimz) {
+    rbgs<float>(x, b, res, t, dimx, dimy, dimz);
+}
+extern "C" void mg_RBGS_periodic3D2(double* x, double* b, double* res, int t, int dimx, int dimy, int dimz) {
+    rbgs<double>(x, b, res, t, dimx, dimy, dimz);
+}
+extern "C" void mg_ComputeResidualPeriodic3D(float* x, float* b, float* r, int dimx, int dimy, int dimz) {
+    report(ref_residual<float>(x, b, r, dimx, dimy, dimz, 0), "mg_ComputeResidualPeriodic3D");
+}
+extern "C" void mg_ComputeResidualPeriodic3D2(double* x, double* b, double* r, int dimx, int dimy, int dimz) {
+    report(ref_residual<double>(x, b, r, dimx, dimy, dimz, 0), "mg_ComputeResidualPeriodic3D2");
+}
+extern "C" void mg_RestrictionPeriodic3D(float* next_level, float* curr_level, int next_dimx, int next_dimy,
+                                         int next_dimz) {
+    report(ref_restrict<float>(next_level, curr_level, next_dimx, next_dimy, next_dimz, 0), "mg_Restriction");
+}
+extern "C" void mg_RestrictionPeriodic3D2(double* next_level, double* curr_level, int next_dimx, int next_dimy,
+                                          int next_dimz) {
+    report(ref_restrict<double>(next_level, curr_level, next_dimx, next_dimy, next_dimz, 0), "mg_Restriction2");
+}
+extern "C" void mg_ProlongationPeriodic3D(float* next_level, float* coaser_level, int next_dimx, int next_dimy,
+                                          int next_dimz) {
+    report(ref_prolong<float>(next_level, coaser_level, next_dimx, next_dimy, next_dimz, 0), "mg_Prolongation");
+}
+extern "C" void mg_ProlongationPeriodic3D2(double* next_level, double* coaser_level, int next_dimx, int next_dimy,
+                                           int next_dimz) {
+    report(ref_prolong<double>(next_level, coaser_level, next_dimx, next_dimy, next_dimz, 0), "mg_Prolongation2");
+}
+
+// ---------------------------------------------------------------------------
+// MemoryOperation.cu:8-75
+extern "C" void InitCuda(int argc, char* argv[]) {
+    // "-device=N" selects a device as the reference's findCudaDevice does
+    int dev = 0;
+    for (int i = 1; i < argc && argv; i++)
+        if (argv[i] && strncmp(argv[i], "-device=", 8) == 0) dev = atoi(argv[i] + 8);
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) {
+        pppm_set_last_error("InitCuda: no HIP device", __FILE__, __LINE__);
+        return;
+    }
+    if (dev < 0 || dev >= nd) dev = 0;
+    if (hipSetDevice(dev) != hipSuccess) pppm_set_last_error("InitCuda: hipSetDevice failed", __FILE__, __LINE__);
+}
+extern "C" void AllocateMemoryPitch(size_t size, size_t* pitch, void** devPtr) {
+    if (hipMallocPitch(devPtr, pitch, size, 1) != hipSuccess)
+        pppm_set_last_error("AllocateMemoryPitch failed", __FILE__, __LINE__);
+}
+extern "C" void AllocateMemory(size_t size, void** devPtr) {
+    if (hipMalloc(devPtr, size) != hipSuccess) pppm_set_last_error("AllocateMemory failed", __FILE__, __LINE__);
+}
+extern "C" void FreeMemory(void* devPtr) {
+    if (devPtr && hipFree(devPtr) != hipSuccess) pppm_set_last_error("FreeMemory failed", __FILE__, __LINE__);
+}
+extern "C" void ThreadSync(void) {
+    if (hipDeviceSynchronize() != hipSuccess) pppm_set_last_error("ThreadSync failed", __FILE__, __LINE__);
+}
+extern "C" void MemcpyHstToDevPitch(void* devPtr, size_t dPitch, const void* hstPtr, size_t hPitch, size_t size) {
+    if (hipMemcpy2D(devPtr, dPitch, hstPtr, hPitch, size, 1, hipMemcpyHostToDevice) != hipSuccess)
+        pppm_set_last_error("MemcpyHstToDevPitch failed", __FILE__, __LINE__);
+}
+extern "C" void MemcpyDevToHstPitch(const void* devPtr, size_t dPitch, void* hstPtr, size_t hPitch, size_t size) {
+    if (hipMemcpy2D(hstPtr, hPitch, devPtr, dPitch, size, 1, hipMemcpyDeviceToHost) != hipSuccess)
+        pppm_set_last_error("MemcpyDevToHstPitch failed", __FILE__, __LINE__);
+}
+extern "C" void MemcpyDevToDevPitch(void* dstPtr, size_t dPitch, const void* srcPtr, size_t sPitch, size_t size) {
+    if (hipMemcpy2D(dstPtr, dPitch, srcPtr, sPitch, size, 1, hipMemcpyDeviceToDevice) != hipSuccess)
+        pppm_set_last_error("MemcpyDevToDevPitch failed", __FILE__, __LINE__);
+}
+extern "C" void MemcpyHstToDev(void* devPtr, const void* hstPtr, size_t size) {
+    if (hipMemcpy(devPtr, hstPtr, size, hipMemcpyHostToDevice) != hipSuccess)
+        pppm_set_last_error("MemcpyHstToDev failed", __FILE__, __LINE__);
+}
+extern "C" void MemcpyDevToHst(const void* devPtr, void* hstPtr, size_t size) {
+    if (hipMemcpy(hstPtr, devPtr, size, hipMemcpyDeviceToHost) != hipSuccess)
+        pppm_set_last_error("MemcpyDevToHst failed", __FILE__, __LINE__);
+}
+extern "C" void MemcpyDevToDev(void* dstPtr, const void* srcPtr, size_t size) {
+    if (hipMemcpy(dstPtr, srcPtr, size, hipMemcpyDeviceToDevice) != hipSuccess)
+        pppm_set_last_error("MemcpyDevToDev failed", __FILE__, __LINE__);
+}

@@ -1,0 +1,348 @@
+"""ctypes binding of libpppm_hip.so (the C ABI in include/pppm.h).
+
+Loads the in-tree library and fails loudly when it is missing; there is no
+CPU fallback anywhere in the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libpppm_hip.so")
+
+PPPM_OK = 0
+STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOMEM", 4: "BUDGET", 5: "NOT_CONVERGED", 6: "STATE", 7: "NO_DEVICE"}
+F32, F64 = 1, 2
+KERNEL_EXPCUBIC, KERNEL_GAUSSIAN = 0, 1
+MG_VCYCLE, MG_REF_FMG = 0, 1
+PREC_FP32, PREC_MIXED = 0, 1
+
+
+class PPPMError(RuntimeError):
+    def __init__(self, code, where):
+        msg = lib().pppm_last_error().decode(errors="replace") if _lib is not None else ""
+        super().__init__(f"{where}: {STATUS.get(code, code)} {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("g", C.c_uint32), ("h", C.c_double), ("origin", C.c_double * 3), ("K", C.c_int32),
+        ("kernel", C.c_int32), ("core_a", C.c_double), ("mg_schedule", C.c_int32),
+        ("nu1", C.c_int32), ("nu2", C.c_int32), ("mg_omega", C.c_double), ("mg_tol", C.c_double),
+        ("mg_max_cycles", C.c_int32), ("precision", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32), ("m", C.c_uint32), ("mg_cycles", C.c_int32), ("mg_outer", C.c_int32),
+        ("mg_relres", C.c_double * 3), ("strays", C.c_uint32), ("mg_converged", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        L.pppm_last_error.restype = C.c_char_p
+        L.pppm_sort_scratch_bytes.restype = C.c_size_t
+        L.pppm_sort_scratch_bytes.argtypes = [C.c_uint32]
+        L.pppm_destroy.restype = None
+        L.pppm_hash_destroy.restype = None
+        L.pppm_mg_destroy.restype = None
+        L.pppm_timer_destroy.restype = None
+        L.pppm_params_config_preset.restype = None
+        L.pppm_params_reference_preset.restype = None
+        _lib = L
+    return _lib
+
+
+def check(code, where=""):
+    if code != PPPM_OK:
+        raise PPPMError(code, where)
+
+
+def device_count():
+    n = C.c_int32(0)
+    lib().pppm_device_count(C.byref(n))
+    return n.value
+
+
+def synchronize():
+    check(lib().pppm_synchronize(), "synchronize")
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (pppm_malloc / pppm_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(lib().pppm_malloc(C.byref(self.ptr), C.c_size_t(max(1, self.nbytes))), "malloc")
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().pppm_memcpy_htod(b.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes)), "htod")
+        return b
+
+    def to_array(self, dtype, count=None, offset_bytes=0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset_bytes) // dtype.itemsize
+        out = np.empty(count, dtype)
+        if out.nbytes:
+            src = C.c_void_p(self.ptr.value + offset_bytes)
+            check(lib().pppm_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), src, C.c_size_t(out.nbytes)), "dtoh")
+        return out
+
+    def fill_zero(self):
+        check(lib().pppm_memset(self.ptr, 0, C.c_size_t(self.nbytes)), "memset")
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def at(self, offset_bytes):
+        return C.c_void_p(self.ptr.value + offset_bytes)
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            lib().pppm_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def dev_to_array(ptr, dtype, count):
+    dtype = np.dtype(dtype)
+    out = np.empty(count, dtype)
+    if out.nbytes:
+        check(lib().pppm_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(out.nbytes)),
+              "dtoh")
+    return out
+
+
+def config_params(g, **kw):
+    p = Params()
+    lib().pppm_params_config_preset(C.byref(p), C.c_uint32(g))
+    for k, v in kw.items():
+        if k == "origin":
+            for i in range(3):
+                p.origin[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def reference_params(g, K, **kw):
+    p = Params()
+    lib().pppm_params_reference_preset(C.byref(p), C.c_uint32(g), C.c_int32(K))
+    for k, v in kw.items():
+        if k == "origin":
+            for i in range(3):
+                p.origin[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def reference_domain(pos4, g):
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    o = (C.c_double * 3)()
+    h = C.c_double()
+    check(lib().pppm_reference_domain(pos4.ctypes.data_as(C.c_void_p), C.c_uint32(pos4.shape[0]), C.c_uint32(g), o,
+                                      C.byref(h)), "reference_domain")
+    return [o[0], o[1], o[2]], h.value
+
+
+class Context:
+    """pppm_ctx: device-resident PPPM state for one scene."""
+
+    def __init__(self, params):
+        self.params = params
+        self.h = C.c_void_p()
+        check(lib().pppm_create(C.byref(params), C.byref(self.h)), "create")
+
+    @property
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().pppm_get_stream(self.h, C.byref(s)), "get_stream")
+        return s
+
+    def set_vortices(self, n, pos_buf, w_bufs, w_stride=1, w_dtype=F64):
+        wx, wy, wz = (C.c_void_p(b) if isinstance(b, int) else b for b in w_bufs)
+        check(lib().pppm_set_vortices(self.h, C.c_uint32(n), pos_buf, wx, wy, wz, C.c_uint32(w_stride),
+                                      C.c_int32(w_dtype)), "set_vortices")
+
+    def compute_far_field(self):
+        check(lib().pppm_compute_far_field(self.h), "compute_far_field")
+
+    def evaluate_self(self, u_bufs, u_stride=1, u_dtype=F64):
+        ux, uy, uz = (C.c_void_p(b) if isinstance(b, int) else b for b in u_bufs)
+        check(lib().pppm_evaluate_self(self.h, ux, uy, uz, C.c_uint32(u_stride), C.c_int32(u_dtype)), "evaluate_self")
+
+    def evaluate(self, m, eval_buf, other_buf, u_bufs, u_stride=1, u_dtype=F64):
+        ux, uy, uz = (C.c_void_p(b) if isinstance(b, int) else b for b in u_bufs)
+        check(lib().pppm_evaluate(self.h, C.c_uint32(m), eval_buf, other_buf, ux, uy, uz, C.c_uint32(u_stride),
+                                  C.c_int32(u_dtype)), "evaluate")
+
+    def stats(self):
+        s = Stats()
+        check(lib().pppm_get_stats(self.h, C.byref(s)), "get_stats")
+        return s
+
+    def sorted_vortices(self):
+        pos, idx, st, en = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().pppm_get_sorted_vortices(self.h, C.byref(pos), C.byref(idx), C.byref(st), C.byref(en)),
+              "get_sorted_vortices")
+        return pos.value, idx.value, st.value, en.value
+
+    def grid(self, which, comp):
+        p, dt = C.c_void_p(), C.c_int32()
+        check(lib().pppm_get_grid(self.h, C.c_int32(which), C.c_int32(comp), C.byref(p), C.byref(dt)), "get_grid")
+        g = self.params.g
+        return dev_to_array(p.value, np.float64 if dt.value == F64 else np.float32, g ** 3)
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().pppm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Host arrays uploaded once: positions (N,4) f32, strengths (3,N)."""
+
+    def __init__(self, pos4, omega, w_dtype=F64):
+        self.n = pos4.shape[0]
+        self.pos = DeviceBuffer.from_array(np.ascontiguousarray(pos4, np.float32))
+        wt = np.float64 if w_dtype == F64 else np.float32
+        self.w_dtype = w_dtype
+        self.w = [DeviceBuffer.from_array(np.ascontiguousarray(omega[i], wt)) for i in range(3)]
+
+    def w_ptrs(self):
+        return [b.ptr for b in self.w]
+
+
+def solve_self(ctx, scene, u_dtype=F64):
+    """Velocity (3, N) at the vortices, host float64 copy."""
+    isz = 8 if u_dtype == F64 else 4
+    out = [DeviceBuffer(scene.n * isz) for _ in range(3)]
+    ctx.set_vortices(scene.n, scene.pos.ptr, scene.w_ptrs(), 1, scene.w_dtype)
+    ctx.compute_far_field()
+    ctx.evaluate_self([b.ptr for b in out], 1, u_dtype)
+    synchronize()
+    dt = np.float64 if u_dtype == F64 else np.float32
+    return np.stack([b.to_array(dt, scene.n).astype(np.float64) for b in out])
+
+
+class Timer:
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().pppm_timer_create(C.byref(self.h)), "timer_create")
+
+    def start(self, stream):
+        check(lib().pppm_timer_start(self.h, stream), "timer_start")
+
+    def stop(self, stream):
+        check(lib().pppm_timer_stop(self.h, stream), "timer_stop")
+
+    def ms(self):
+        v = C.c_float()
+        check(lib().pppm_timer_elapsed_ms(self.h, C.byref(v)), "timer_elapsed")
+        return v.value
+
+    def __del__(self):
+        try:
+            lib().pppm_timer_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Hash:
+    """pppm_hash: standalone spatial hash (ParticleManager)."""
+
+    def __init__(self, capacity, g, h, origin):
+        self.h = C.c_void_p()
+        o = (C.c_float * 3)(*[float(np.float32(v)) for v in origin])
+        check(lib().pppm_hash_create(C.c_uint32(capacity), C.c_uint32(g), C.c_float(h), o, C.byref(self.h)),
+              "hash_create")
+        self.g = g
+
+    def build(self, pos_buf, n, stream=None):
+        check(lib().pppm_hash_build(self.h, pos_buf, C.c_uint32(n), stream), "hash_build")
+
+    def tables(self, n):
+        ks, vs, st, en = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().pppm_hash_tables(self.h, C.byref(ks), C.byref(vs), C.byref(st), C.byref(en)), "hash_tables")
+        G = self.g ** 3
+        synchronize()
+        return (dev_to_array(ks.value, np.uint32, n), dev_to_array(vs.value, np.uint32, n),
+                dev_to_array(st.value, np.uint32, G), dev_to_array(en.value, np.uint32, G))
+
+    def reorder(self, n, in_buf, out_buf, stride, dtype, stream=None):
+        check(lib().pppm_hash_reorder(self.h, C.c_uint32(n), in_buf, out_buf, C.c_uint32(stride), C.c_int32(dtype),
+                                      stream), "hash_reorder")
+
+    def deorder(self, n, in_buf, out_buf, stride, dtype, stream=None):
+        check(lib().pppm_hash_deorder(self.h, C.c_uint32(n), in_buf, out_buf, C.c_uint32(stride), C.c_int32(dtype),
+                                      stream), "hash_deorder")
+
+    def __del__(self):
+        try:
+            lib().pppm_hash_destroy(self.h)
+        except Exception:
+            pass
+
+
+class MG:
+    """pppm_mg: standalone multigrid on caller grids."""
+
+    def __init__(self, dims, dtype=F64):
+        self.h = C.c_void_p()
+        self.dims = tuple(int(d) for d in dims)
+        self.dtype = dtype
+        check(lib().pppm_mg_create(*[C.c_int32(d) for d in self.dims], C.c_int32(dtype), C.byref(self.h)),
+              "mg_create")
+
+    def levels(self):
+        L = C.c_int32()
+        dims = (C.c_int32 * 3 * 16)()
+        check(lib().pppm_mg_levels(self.h, C.byref(L), dims, C.c_int32(16)), "mg_levels")
+        return [tuple(dims[i]) for i in range(L.value)]
+
+    def solve(self, x_buf, b_buf, schedule=MG_VCYCLE, nu1=2, nu2=2, omega=1.0, tol=1e-6, max_cycles=50,
+              stream=None):
+        cyc = C.c_int32()
+        rel = C.c_double()
+        code = lib().pppm_mg_solve(self.h, x_buf, b_buf, C.c_int32(schedule), C.c_int32(nu1), C.c_int32(nu2),
+                                   C.c_double(omega), C.c_double(tol), C.c_int32(max_cycles), C.byref(cyc),
+                                   C.byref(rel), stream)
+        if code not in (PPPM_OK, 5):
+            check(code, "mg_solve")
+        return cyc.value, rel.value, code
+
+    def __del__(self):
+        try:
+            lib().pppm_mg_destroy(self.h)
+        except Exception:
+            pass
