@@ -1,0 +1,188 @@
+"""PPPM vortex-velocity benchmark on one MI355X.
+
+Workload (default): buoyant plume, 8,388,608 vortices on a 256^3 grid,
+K = 2, expcubic core 1.5 h, V(2,2) multigrid.  One step = one full velocity
+evaluation at every vortex: hash + sort + reorder, P2M, three Poisson solves,
+curl + far-field buffer, M2P + near field + unsort.  Inputs (float4 positions,
+float32 strengths) are resident in HBM before the timed region.
+
+Two paths are timed on the same inputs:
+  mixed : FP32 multigrid inside FP64 defect correction (tolerance 1e-9,
+          velocity error <= 1e-5 vs the FP64 oracle in the parity tests)
+  fp32  : plain FP32 path (V-cycles to 1e-6 or the FP32 rounding floor)
+`value` is the mixed path's particle-velocities/s; the plain FP32 figure and
+per-step p50/p99 latencies are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+FLOP_PER_PAIR = 32
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plume-8M-256")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--paths", default="mixed,fp32")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(N, scene, g, prec, steps, warmup):
+    p = N.config_params(g, precision=prec, mg_tol=1e-9 if prec == N.PREC_MIXED else 1e-6,
+                        mg_max_cycles=40)
+    ctx = N.Context(p)
+    st = ctx.stream
+    n = scene.n
+    out = [N.DeviceBuffer(4 * n) for _ in range(3)]
+    t_hash, t_far, t_near = N.Timer(), N.Timer(), N.Timer()
+
+    def step(timed):
+        if timed:
+            t_hash.start(st)
+        ctx.set_vortices(n, scene.pos.ptr, scene.w_ptrs(), 1, scene.w_dtype)
+        if timed:
+            t_hash.stop(st)
+            t_far.start(st)
+        ctx.compute_far_field()
+        if timed:
+            t_far.stop(st)
+            t_near.start(st)
+        ctx.evaluate_self([b.ptr for b in out], 1, N.F32)
+        if timed:
+            t_near.stop(st)
+
+    for _ in range(warmup):
+        step(False)
+    N.synchronize()
+    lat, th, tf, tn = [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        step(True)
+        N.synchronize()
+        lat.append(time.perf_counter() - s0)
+        th.append(t_hash.ms())
+        tf.append(t_far.ms())
+        tn.append(t_near.ms())
+    N.synchronize()
+    wall = time.perf_counter() - t0
+    stats = ctx.stats()
+    pairs = ctx.h and _count_pairs(N, ctx)
+    ctx.close()
+    return {
+        "wall_s": wall, "lat_ms": [1e3 * x for x in lat], "hash_ms": th, "far_ms": tf, "near_ms": tn,
+        "mg_cycles": stats.mg_cycles, "mg_outer": stats.mg_outer, "mg_relres": list(stats.mg_relres),
+        "mg_converged": bool(stats.mg_converged), "pairs": pairs,
+    }
+
+
+def _count_pairs(N, ctx):
+    import ctypes as C
+
+    v = C.c_uint64()
+    N.check(N.lib().pppm_count_near_pairs(ctx.h, C.byref(v)), "count_near_pairs")
+    return int(v.value)
+
+
+def cpu_baseline():
+    """The oracle (FP64 CPU restatement, OpenMP over output elements) on a
+    bounded sample of the same workload: the plume distribution at the same
+    vortices per cell, 1,048,576 vortices on 128^3."""
+    from oracle import oracle_py as O
+    from pppm_vortex_bouyant_flow_sample_amd import scenes
+
+    n, g = 1048576, 128
+    pos, w, _ = scenes.plume(n)
+    w = w.astype(np.float32).astype(np.float64)
+    p = O.make_params(g, 1.0 / g, [0, 0, 0], K=2, tol=1e-6, max_cycles=40)
+    t = time.perf_counter()
+    O.pppm_eval(p, pos, w)
+    t = time.perf_counter() - t
+    return {"value": n / t, "unit": "particle-velocities/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"plume, {n} vortices on {g}^3 (same vortices/cell as 8M on 256^3), FP64, {t:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {a.gpus} requested); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from pppm_vortex_bouyant_flow_sample_amd import native as N
+    from pppm_vortex_bouyant_flow_sample_amd import scenes
+
+    if N.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(3)
+    pos, w, g = scenes.make(a.workload)
+    n = pos.shape[0]
+    log(f"[bench] workload {a.workload}: {n} vortices, {g}^3 grid")
+    scene = N.Scene(pos, w.astype(np.float32), N.F32)
+    res = {}
+    for name in a.paths.split(","):
+        prec = N.PREC_MIXED if name == "mixed" else N.PREC_FP32
+        log(f"[bench] path {name}: warmup {a.warmup}, steps {a.steps}")
+        res[name] = run_path(N, scene, g, prec, a.steps, a.warmup)
+        r = res[name]
+        log(f"[bench]   {1e3 * r['wall_s'] / a.steps:.2f} ms/step; hash {np.median(r['hash_ms']):.2f} "
+            f"far {np.median(r['far_ms']):.2f} near {np.median(r['near_ms']):.2f} ms; "
+            f"mg cycles {r['mg_cycles']} outer {r['mg_outer']} relres {max(r['mg_relres']):.2e}")
+    head = "mixed" if "mixed" in res else list(res)[0]
+    r = res[head]
+    ms = 1e3 * r["wall_s"] / a.steps
+    value = n / (r["wall_s"] / a.steps)
+    t_near = float(np.mean(r["near_ms"])) * 1e-3
+    achieved = FLOP_PER_PAIR * r["pairs"] / t_near / 1e12
+    out = {
+        "metric": "PPPM particle-velocities/s at 8M/256^3 vs plain FP32 GPU; p50/p99 latency",
+        "value": value, "unit": "particle-velocities/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f32", "data": "synthetic",
+        "config": {"workload": f"{a.workload} (plume: x,y~N(0.5,0.08), z~Exp(3), w~N(0,1)e^-3z)",
+                   "vortices": n, "grid": g, "K": 2, "core": "expcubic 1.5h", "multigrid": "V(2,2) RB-GS",
+                   "path": head, "parallelism": "1 GPU"},
+        "latency_ms": {"p50": float(np.percentile(r["lat_ms"], 50)), "p99": float(np.percentile(r["lat_ms"], 99))},
+        "stages_ms": {"hash": float(np.median(r["hash_ms"])), "p2m_mg_far": float(np.median(r["far_ms"])),
+                      "m2p_near": float(np.median(r["near_ms"]))},
+        "mg": {"cycles": r["mg_cycles"], "outer": r["mg_outer"], "relres": max(r["mg_relres"])},
+        "roofline": {"bound": "valu", "kernel": "k_m2p_near", "achieved": achieved,
+                     "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
+                     "pairs": r["pairs"], "flop_per_pair": FLOP_PER_PAIR},
+    }
+    for other in res:
+        if other != head:
+            ro = res[other]
+            out[f"{other}_path"] = {
+                "value": n / (ro["wall_s"] / a.steps), "ms_per_step": 1e3 * ro["wall_s"] / a.steps,
+                "latency_ms": {"p50": float(np.percentile(ro["lat_ms"], 50)),
+                               "p99": float(np.percentile(ro["lat_ms"], 99))},
+                "mg": {"cycles": ro["mg_cycles"], "relres": max(ro["mg_relres"]), "converged": ro["mg_converged"]},
+            }
+    if not a.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

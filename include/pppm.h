@@ -46,14 +46,15 @@ typedef enum pppm_status {
  * (particle_hash_kernel.cu:143-152: 1 = float, 2 = double) */
 enum { PPPM_F32 = 1, PPPM_F64 = 2 };
 enum { PPPM_KERNEL_EXPCUBIC = 0, PPPM_KERNEL_GAUSSIAN = 1 };
-enum { PPPM_MG_VCYCLE = 0, PPPM_MG_REF_FMG = 1 };
+enum { PPPM_MG_VCYCLE = 0, PPPM_MG_REF_FMG = 1, PPPM_MG_REF_VCYCLE = 2 };
 enum { PPPM_PREC_FP32 = 0, PPPM_PREC_MIXED = 1 };
 
 typedef struct pppm_params {
     uint32_t g;             /* grid cells per axis (cubic grid and hash), 8..1024      */
     double h;               /* cell size; the hash keeps (float)h, the path (double)(float)h */
     double origin[3];       /* domain origin; the hash keeps (float)origin              */
-    int32_t K;              /* near-field range in cells, 0..4                          */
+    int32_t K;              /* near-field range in cells, 0..4; -1 = vortex-in-cell
+                               (no near field, M2P of the grid velocity)           */
     int32_t kernel;         /* PPPM_KERNEL_*                                           */
     double core_a;          /* smoothing core radius (absolute units)                 */
     int32_t mg_schedule;    /* PPPM_MG_VCYCLE (to tolerance) or PPPM_MG_REF_FMG          */
@@ -91,6 +92,9 @@ void pppm_destroy(pppm_ctx *ctx);
 pppm_status pppm_set_stream(pppm_ctx *ctx, void *hip_stream);
 pppm_status pppm_get_stream(const pppm_ctx *ctx, void **hip_stream);
 pppm_status pppm_get_params(const pppm_ctx *ctx, pppm_params *out);
+/* Move the domain (origin, h) of an existing context without reallocating;
+ * the grid size stays.  Invalidates the hashed vortices and the far field. */
+pppm_status pppm_set_domain(pppm_ctx *ctx, const double origin[3], double h);
 
 /* Vortices: pos4 = float4 rows (x, y, z, unused); strength component c of
  * vortex i at w_c[i * w_stride] with element type w_dtype (PPPM_F32/PPPM_F64).
@@ -111,6 +115,10 @@ pppm_status pppm_solve_self(pppm_ctx *ctx, uint32_t n, const float *pos4, const 
                             const void *wz, uint32_t w_stride, int32_t w_dtype, void *ux, void *uy, void *uz,
                             uint32_t u_stride, int32_t u_dtype);
 pppm_status pppm_get_stats(const pppm_ctx *ctx, pppm_stats *out);
+/* Near-field pairs of the vortices with each other that pass the box test
+ * |dx|,|dy|,|dz| < (K + 1/2) h and are not coincident (self-evaluation),
+ * counted on the device; used for FLOP accounting (synchronous). */
+pppm_status pppm_count_near_pairs(pppm_ctx *ctx, uint64_t *pairs);
 
 /* Sorted device views of the last set_vortices (BiotSavartSolver::getSortPos /
  * getVortStartTable / getVortEndTable / getSortVort, BiotSavartSolver.h:25-29).
@@ -161,6 +169,8 @@ void pppm_mg_destroy(pppm_mg *mg);
 pppm_status pppm_mg_levels(const pppm_mg *mg, int32_t *levels, int32_t dims_out[][3], int32_t max_levels);
 /* Solve sum_nb(x) - 6 x = b (zero ghosts) in place on x.  schedule
  * PPPM_MG_REF_FMG = one full-multigrid pass + one V-cycle (the reference);
+ * PPPM_MG_REF_VCYCLE = one V-cycle with the reference's transfers
+ * (MultiGrid.cpp:128-201 from level 0);
  * PPPM_MG_VCYCLE = V(nu1,nu2) cycles until ||b - A x|| / ||b|| <= tol.
  * cycles_out / relres_out may be NULL. */
 pppm_status pppm_mg_solve(pppm_mg *mg, void *x, const void *b, int32_t schedule, int32_t nu1, int32_t nu2,

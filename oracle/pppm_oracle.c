@@ -1086,7 +1086,11 @@ int orc_pppm_eval(const orc_params *p, uint32_t n, const float *vort_pos4,
         far[c] = (double *)malloc(sizeof(double) * G);
     }
     orc_curl(psi[0], psi[1], psi[2], U[0], U[1], U[2], (int)g, (int)g, (int)g, cell_h);
-    orc_far_field(wg[0], wg[1], wg[2], U[0], U[1], U[2], far[0], far[1], far[2], cell_h, (int)g, p->K);
+    if (p->K >= 0) {
+        orc_far_field(wg[0], wg[1], wg[2], U[0], U[1], U[2], far[0], far[1], far[2], cell_h, (int)g, p->K);
+    } else { /* vortex-in-cell (BiotSavartSolverImp.cpp:487-497): interpolate grid_U */
+        for (c = 0; c < 3; c++) memcpy(far[c], U[c], sizeof(double) * G);
+    }
     t1 = wtime();
     if (diag) diag->t_far = t1 - t0;
 

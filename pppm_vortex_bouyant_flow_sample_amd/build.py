@@ -60,7 +60,28 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_dropin_test()
     return LIB
+
+
+def build_dropin_test():
+    """The C++ program that drives the drop-in classes (tests/dropin), built
+    with the host compiler against include/ and the library."""
+    src = os.path.join(ROOT, "tests", "dropin", "dropin_main.cpp")
+    if not os.path.exists(src):
+        return None
+    out_dir = os.path.join(ROOT, "tests", "dropin", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "dropin_main")
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I", os.path.join(ROOT, "include"), src, "-o", exe + ".tmp",
+           "-L", OUT_DIR, "-lpppm_hip", "-Wl,-rpath," + OUT_DIR,
+           "-Wl,-rpath,$ORIGIN/../../../pppm_vortex_bouyant_flow_sample_amd/_build"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in test program failed to build:\n" + r.stdout + r.stderr)
+    os.replace(exe + ".tmp", exe)
+    return exe
 
 
 if __name__ == "__main__":

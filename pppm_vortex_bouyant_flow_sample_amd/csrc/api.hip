@@ -90,7 +90,7 @@ pppm_status validate_params(const pppm_params* p) {
     if (!p) FAIL(PPPM_ERR_INVALID, "null params");
     if (p->g < 2 || p->g > 1024) FAIL(PPPM_ERR_INVALID, "g must be in [2, 1024]");
     if (!(p->h > 0) || !std::isfinite(p->h)) FAIL(PPPM_ERR_INVALID, "h must be positive");
-    if (p->K < 0 || p->K > 4) FAIL(PPPM_ERR_INVALID, "K must be in [0, 4]");
+    if (p->K < -1 || p->K > 4) FAIL(PPPM_ERR_INVALID, "K must be in [-1, 4]");
     if (p->kernel != PPPM_KERNEL_EXPCUBIC && p->kernel != PPPM_KERNEL_GAUSSIAN) FAIL(PPPM_ERR_INVALID, "kernel");
     if (!(p->core_a > 0)) FAIL(PPPM_ERR_INVALID, "core_a must be positive");
     if (p->mg_schedule != PPPM_MG_VCYCLE && p->mg_schedule != PPPM_MG_REF_FMG) FAIL(PPPM_ERR_INVALID, "schedule");
@@ -200,10 +200,17 @@ struct pppm_ctx {
         } else {
             TRY(dev_alloc(&psi32, 3 * G));
         }
-        std::vector<float> dt = far_dir_table(p.K, pg.h);
-        TRY(dev_alloc(&dir, dt.size()));
-        PPPM_HIP_TRY(hipMemcpy(dir, dt.data(), dt.size() * sizeof(float), hipMemcpyHostToDevice));
+        TRY(dev_alloc(&dir, 3 * 9 * 9 * 9));
+        TRY(upload_dir());
         TRY(mg.init((int)g, (int)g, (int)g, 3));
+        return PPPM_OK;
+    }
+
+    pppm_status upload_dir() {
+        if (p.K < 0) return PPPM_OK;
+        std::vector<float> dt = far_dir_table(p.K, pg.h);
+        PPPM_HIP_TRY(hipMemcpyAsync(dir, dt.data(), dt.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        PPPM_HIP_TRY(hipStreamSynchronize(st));
         return PPPM_OK;
     }
 
@@ -350,6 +357,20 @@ extern "C" pppm_status pppm_get_stream(const pppm_ctx* c, void** s) {
     if (!c || !s) FAIL(PPPM_ERR_INVALID, "null arg");
     *s = (void*)c->st;
     return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_set_domain(pppm_ctx* c, const double origin[3], double h) {
+    if (!c || !origin) FAIL(PPPM_ERR_INVALID, "null arg");
+    pppm_params np = c->p;
+    np.h = h;
+    for (int i = 0; i < 3; i++) np.origin[i] = origin[i];
+    TRY(validate_params(&np));
+    PPPM_HIP_TRY(hipStreamSynchronize(c->st));
+    c->p = np;
+    c->pg = make_geom(np);
+    c->have_vort = false;
+    c->have_far = false;
+    return c->upload_dir();
 }
 
 extern "C" pppm_status pppm_get_params(const pppm_ctx* c, pppm_params* out) {
@@ -530,6 +551,31 @@ extern "C" pppm_status pppm_solve_self(pppm_ctx* c, uint32_t n, const float* pos
     TRY(pppm_set_vortices(c, n, pos4, wx, wy, wz, w_stride, w_dtype));
     TRY(pppm_compute_far_field(c));
     return pppm_evaluate_self(c, ux, uy, uz, u_stride, u_dtype);
+}
+
+extern "C" pppm_status pppm_count_near_pairs(pppm_ctx* c, uint64_t* pairs) {
+    if (!c || !pairs) FAIL(PPPM_ERR_INVALID, "null arg");
+    if (!c->have_vort) FAIL(PPPM_ERR_STATE, "set_vortices first");
+    unsigned long long* d = nullptr;
+    PPPM_HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+    NearArgs a{};
+    a.epos = c->spos;
+    a.m = c->n;
+    a.off = c->coff;
+    a.spos = c->spos;
+    a.g = c->g;
+    a.K = c->p.K;
+    a.invh = c->pg.invh;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long), c->st);
+    pppm_status s = e == hipSuccess ? launch_count_pairs(a, d, c->st) : PPPM_ERR_HIP;
+    unsigned long long h = 0;
+    if (s == PPPM_OK) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->st);
+    if (s == PPPM_OK && e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(d);
+    if (s != PPPM_OK) return s;
+    PPPM_HIP_TRY(e);
+    *pairs = h;
+    return PPPM_OK;
 }
 
 extern "C" pppm_status pppm_get_stats(const pppm_ctx* c, pppm_stats* out) {

@@ -76,7 +76,8 @@ pppm_status launch_curl_far_impl(const void* psi, int psi_dtype, const float* wg
     const uint32_t G = g * g * g;
     const float h3 = (float)(h * h * h);
     const double half_invh = 0.5 * (1.0 / h);
-    const float cd = (float)(kRefDiagPsi * h * h * half_invh);
+    // vortex-in-cell (K < 0): the far-field buffer is the grid velocity itself
+    const float cd = K < 0 ? 0.0f : (float)(kRefDiagPsi * h * h * half_invh);
     if (psi_dtype == PPPM_F64)
         hipLaunchKernelGGL((k_curl_far<double, double>), dim3(div_up(G, 256)), dim3(256), 0, st, (const double*)psi,
                            wgrid, far, dir, g, K, h3, half_invh, cd);

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
     const int R = K + 1;
     const uint32_t nsrc = off[G];
     const int ilo = max(i - R, 0), ihi = min(i + R, gi - 1);
-    if (ilo <= ihi) {
+    if (K >= 0 && ilo <= ihi) {
         for (int kk = max(k - R, 0); kk <= min(k + R, gi - 1); kk++) {
             for (int jj = max(j - R, 0); jj <= min(j + R, gi - 1); jj++) {
                 const uint32_t rowc = ((uint32_t)kk * g + (uint32_t)jj) * g;
@@ -141,8 +141,9 @@ static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st)
                            a.u_stride);
 }
 
-// largest float f with (double)f < x (x > 0)
+// largest float f with (double)f < x (x > 0; returns -1 otherwise)
 static float float_below(double x) {
+    if (!(x > 0)) return -1.0f;
     float f = (float)x;
     while ((double)f >= x) f = nextafterf(f, 0.0f);
     while ((double)nextafterf(f, 3.4e38f) < x) f = nextafterf(f, 3.4e38f);
@@ -166,6 +167,49 @@ pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st) {
         if (seg) launch_typed<PPPM_KERNEL_EXPCUBIC, true>(a, nc, st);
         else launch_typed<PPPM_KERNEL_EXPCUBIC, false>(a, nc, st);
     }
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+// Accepted near-field pairs (box test passed, r^2 >= 1e-24) of the evaluation
+// points against the vortices, for FLOP accounting; one atomic per wave.
+__global__ void __launch_bounds__(256) k_count_pairs(const float4* __restrict__ epos, uint32_t m,
+                                                     const uint32_t* __restrict__ off, const float4* __restrict__ spos,
+                                                     uint32_t g, int K, double invh, float rbox,
+                                                     unsigned long long* __restrict__ total) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (t < m) {
+        const int gi = (int)g;
+        const size_t G = (size_t)g * g * g;
+        const float4 a = epos[t];
+        const int i = cell_of(a.x, invh), j = cell_of(a.y, invh), k = cell_of(a.z, invh);
+        const int R = K + 1;
+        const uint32_t nsrc = off[G];
+        const int ilo = max(i - R, 0), ihi = min(i + R, gi - 1);
+        if (K >= 0 && ilo <= ihi)
+            for (int kk = max(k - R, 0); kk <= min(k + R, gi - 1); kk++)
+                for (int jj = max(j - R, 0); jj <= min(j + R, gi - 1); jj++) {
+                    const uint32_t rowc = ((uint32_t)kk * g + (uint32_t)jj) * g;
+                    const uint32_t p0 = off[rowc + ilo], p1 = min(off[rowc + ihi + 1], nsrc);
+                    for (uint32_t p = p0; p < p1; p++) {
+                        const float4 s = spos[p];
+                        const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
+                        if (fabsf(dx) <= rbox && fabsf(dy) <= rbox && fabsf(dz) <= rbox &&
+                            dx * dx + dy * dy + dz * dz >= 1e-24f)
+                            cnt++;
+                    }
+                }
+    }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, cnt);
+}
+
+pppm_status launch_count_pairs(const NearArgs& a, unsigned long long* d_total, hipStream_t st) {
+    if (a.m == 0) return PPPM_OK;
+    float rbox = float_below(((double)a.K + 0.5) / a.invh);
+    hipLaunchKernelGGL(k_count_pairs, dim3(div_up(a.m, 256)), dim3(256), 0, st, a.epos, a.m, a.off, a.spos, a.g, a.K,
+                       a.invh, rbox, d_total);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }

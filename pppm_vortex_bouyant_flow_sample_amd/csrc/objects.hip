@@ -250,8 +250,11 @@ static pppm_status mg_solve_t(MGHier<T>& h, T* x, const T* b, int32_t schedule, 
     double rel[3] = {0, 0, 0};
     int cyc = 0;
     pppm_status s;
-    if (schedule == PPPM_MG_REF_FMG) {
-        TRY(h.ref_fmg(x, b, (T)1, nu1, nu2, (T)omega, st));
+    if (schedule == PPPM_MG_REF_FMG || schedule == PPPM_MG_REF_VCYCLE) {
+        if (schedule == PPPM_MG_REF_FMG)
+            TRY(h.ref_fmg(x, b, (T)1, nu1, nu2, (T)omega, st));
+        else
+            TRY(h.vcycle(x, b, (T)1, nu1, nu2, (T)omega, false, st));
         double rn, bn;
         TRY(h.norms(x, b, (T)1, &rn, &bn, st));
         rel[0] = bn > 0 ? rn / bn : rn;
@@ -270,7 +273,8 @@ extern "C" pppm_status pppm_mg_solve(pppm_mg* m, void* x, const void* b, int32_t
                                      double omega, double tol, int32_t max_cycles, int32_t* cycles_out,
                                      double* relres_out, void* stream) {
     if (!m || !x || !b) FAIL(PPPM_ERR_INVALID, "null arg");
-    if (schedule != PPPM_MG_VCYCLE && schedule != PPPM_MG_REF_FMG) FAIL(PPPM_ERR_INVALID, "schedule");
+    if (schedule != PPPM_MG_VCYCLE && schedule != PPPM_MG_REF_FMG && schedule != PPPM_MG_REF_VCYCLE)
+        FAIL(PPPM_ERR_INVALID, "schedule");
     if (nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || max_cycles < 1) FAIL(PPPM_ERR_INVALID, "cycle parameters");
     hipStream_t st = (hipStream_t)stream;
     if (m->dtype == PPPM_F32)

@@ -86,5 +86,6 @@ struct NearArgs {
     int u_dtype;
 };
 pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st);
+pppm_status launch_count_pairs(const NearArgs& a, unsigned long long* d_total, hipStream_t st);
 
 }  // namespace pppm
