@@ -95,6 +95,13 @@ pppm_status pppm_get_params(const pppm_ctx *ctx, pppm_params *out);
 /* Move the domain (origin, h) of an existing context without reallocating;
  * the grid size stays.  Invalidates the hashed vortices and the far field. */
 pppm_status pppm_set_domain(pppm_ctx *ctx, const double origin[3], double h);
+/* Replace all parameters except g and precision (which size the context). */
+pppm_status pppm_update_params(pppm_ctx *ctx, const pppm_params *p);
+/* Device memory the context holds. */
+pppm_status pppm_get_device_bytes(const pppm_ctx *ctx, size_t *bytes);
+/* on = 1 (default): each multigrid iteration (V-cycle plus residual/norm work)
+ * is captured once into a HIP graph and replayed. */
+pppm_status pppm_set_graphs(pppm_ctx *ctx, int32_t on);
 
 /* Vortices: pos4 = float4 rows (x, y, z, unused); strength component c of
  * vortex i at w_c[i * w_stride] with element type w_dtype (PPPM_F32/PPPM_F64).
@@ -122,13 +129,78 @@ pppm_status pppm_count_near_pairs(pppm_ctx *ctx, uint64_t *pairs);
 
 /* Sorted device views of the last set_vortices (BiotSavartSolver::getSortPos /
  * getVortStartTable / getVortEndTable / getSortVort, BiotSavartSolver.h:25-29).
- * Positions are float4 (x, y, z, w) in the caller's coordinates. */
+ * Positions are float4 (x - ox, y - oy, z - oz, w): coordinates relative to
+ * the grid origin (FP32 of the FP64 difference), and w holds the bits of the
+ * uint32 index of the cell the vortex deposits into (0xFFFFFFFF if outside). */
 pppm_status pppm_get_sorted_vortices(pppm_ctx *ctx, const float **pos4, const uint32_t **sorted_index,
                                      const uint32_t **cell_start, const uint32_t **cell_end);
 /* Grid fields after compute_far_field: which = 0 vorticity (h^-3 scaled),
  * 1 stream function, 2 far-field velocity; component c; float32 [g^3]
  * (mixed precision: the stream function is float64). */
 pppm_status pppm_get_grid(pppm_ctx *ctx, int32_t which, int32_t c, const void **ptr, int32_t *dtype);
+
+/* Device bytes a fresh context with these parameters needs for n vortices and
+ * m evaluation points (m = 0: self evaluation; other_end: segment mode). */
+pppm_status pppm_estimate_device_bytes(const pppm_params *p, uint32_t n, uint32_t m, int32_t other_end,
+                                       size_t *bytes);
+
+/* ---- serving: continuous batching of evaluation requests on one GPU -------
+ * No counterpart in the reference, which evaluates one scene at a time from
+ * one host thread (VortexSimulationSystemImp.cpp:142-228, 1194-1244).
+ * Requests run on `slots` worker threads, each on its own resident context and
+ * stream, so their work overlaps on the device.  Contexts stay resident after
+ * a request and are reused by the next request with the same g and precision.
+ * Admission is FIFO against hbm_budget_bytes (least-recently-used idle
+ * contexts are evicted to make room).  Buffers in a request are device
+ * pointers owned by the caller: inputs complete before submit, all valid
+ * until the request is done. */
+typedef struct pppm_server pppm_server;
+typedef struct pppm_server_config {
+    size_t hbm_budget_bytes; /* device bytes the server's contexts may hold          */
+    int32_t slots;           /* requests on the device at once, 1..16               */
+    int32_t use_graphs;      /* capture multigrid iterations into HIP graphs        */
+} pppm_server_config;
+typedef struct pppm_request {
+    pppm_params params;      /* grid, domain, kernel, schedule, precision            */
+    uint32_t n;              /* vortices                                             */
+    const float *pos4;       /* float4[n]                                            */
+    const void *wx, *wy, *wz;
+    uint32_t w_stride;
+    int32_t w_dtype;         /* PPPM_F32 / PPPM_F64                                   */
+    uint32_t m;              /* evaluation points; 0 = at the vortices               */
+    const float *eval_pos4;  /* float4[m] (m > 0)                                    */
+    const float *other_end4; /* segment mode (may be NULL)                           */
+    void *ux, *uy, *uz;      /* velocity out, caller order                           */
+    uint32_t u_stride;
+    int32_t u_dtype;
+} pppm_request;
+typedef struct pppm_request_result {
+    int32_t status;          /* pppm_status of the request                          */
+    int32_t pad;
+    double latency_ms;       /* submit to completion                                 */
+    double queue_ms;         /* submit to start                                      */
+    pppm_stats stats;        /* solver statistics of the request                     */
+} pppm_request_result;
+typedef struct pppm_server_stats {
+    uint64_t submitted, completed, failed, particles;
+    uint32_t queued, running, contexts, pad;
+    uint64_t ctx_created, ctx_reused, ctx_evicted;
+    uint64_t device_bytes, peak_device_bytes;
+    double p50_ms, p99_ms, mean_ms; /* request latency (nearest rank)              */
+    double wall_s;                  /* first submit to last completion             */
+    double particles_per_s;         /* evaluated points / wall_s                   */
+} pppm_server_stats;
+pppm_status pppm_server_create(const pppm_server_config *cfg, pppm_server **out);
+/* Waits for every queued and running request, then frees all contexts. */
+void pppm_server_destroy(pppm_server *srv);
+pppm_status pppm_request_estimate_bytes(const pppm_request *r, size_t *bytes);
+pppm_status pppm_server_submit(pppm_server *srv, const pppm_request *r, uint64_t *ticket);
+pppm_status pppm_server_poll(pppm_server *srv, uint64_t ticket, int32_t *done);
+/* Blocks until the request is done; releases the ticket. */
+pppm_status pppm_server_wait(pppm_server *srv, uint64_t ticket, pppm_request_result *res);
+pppm_status pppm_server_drain(pppm_server *srv);
+pppm_status pppm_server_get_stats(pppm_server *srv, pppm_server_stats *out);
+pppm_status pppm_server_reset_stats(pppm_server *srv);
 
 /* ---- particle hash (ParticleManager, SpaceHashClass.h:10-84) -------------- */
 typedef struct pppm_hash pppm_hash;
@@ -176,6 +248,15 @@ pppm_status pppm_mg_levels(const pppm_mg *mg, int32_t *levels, int32_t dims_out[
 pppm_status pppm_mg_solve(pppm_mg *mg, void *x, const void *b, int32_t schedule, int32_t nu1, int32_t nu2,
                           double omega, double tol, int32_t max_cycles, int32_t *cycles_out,
                           double *relres_out, void *hip_stream);
+
+/* iters red-black SOR iterations (odd colour first, MultiGrid_kernel.cu:722-820)
+ * on the finest level, in place on x; blocked = 1 uses the temporally blocked
+ * LDS smoother (PPPM_F32 only), 0 one launch per half-sweep. */
+pppm_status pppm_mg_smooth(pppm_mg *mg, void *x, const void *b, int32_t iters, double omega, int32_t blocked,
+                           void *hip_stream);
+/* blocked = 1 (default): PPPM_F32 V-cycles use the fused temporally blocked
+ * passes; 0: one launch per half-sweep, residual, restriction, prolongation. */
+pppm_status pppm_mg_set_blocked(pppm_mg *mg, int32_t blocked);
 
 /* ---- device memory (MemoryOperation.cuh:13-29 semantics) -------------------- */
 pppm_status pppm_device_count(int32_t *n);

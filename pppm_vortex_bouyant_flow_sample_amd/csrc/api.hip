@@ -147,6 +147,25 @@ pppm_status launch_curl_far_impl(const void* psi, int psi_dtype, const float* wg
                                  uint32_t g, int K, double h, hipStream_t st);
 }
 
+// Device bytes of a context: the allocations of pppm_ctx::init, plus the
+// vortex buffers for capacity capN and the evaluation buffers for capM (0 =
+// none), as ensure_vort / ensure_eval allocate them.
+static size_t ctx_bytes(const pppm_params& p, uint32_t capN, uint32_t capM) {
+    const size_t G = (size_t)p.g * p.g * p.g;
+    size_t b = 0;
+    b += 4 * G * 2 + 4 * (G + 1) + 4 * (scan_scratch_words(G) + 16) + 16;  // tables, scan, stray counter
+    b += 4 * 3 * G * 2;                                                      // vorticity, far field
+    if (p.precision == PPPM_PREC_MIXED)
+        b += 3 * G * (8 + 8 + 8 + 4 + 4);  // psi64 x2, b64, r32, e32
+    else
+        b += 3 * G * 4;  // psi32
+    b += 4 * 3 * 9 * 9 * 9;  // direction table
+    b += MGHier<float>::bytes_for((int)p.g, (int)p.g, (int)p.g, 3);
+    if (capN) b += (size_t)capN * (4 * 4 + 16 + 16) + sort_scratch_bytes(capN);
+    if (capM) b += (size_t)capM * (4 * 4 + 16 + 16) + sort_scratch_bytes(capM);
+    return b;
+}
+
 struct pppm_ctx {
     pppm_params p{};
     PathGeom pg{};
@@ -174,6 +193,39 @@ struct pppm_ctx {
     MGHier<float> mg;
     bool have_far = false;
     pppm_stats stats{};
+    // per-iteration multigrid work captured once per geometry and replayed
+    bool use_graphs = true;
+    hipGraphExec_t g_mixed[2] = {nullptr, nullptr};
+    hipGraphExec_t g_plain = nullptr;
+
+    void drop_graphs() {
+        for (hipGraphExec_t* e : {&g_mixed[0], &g_mixed[1], &g_plain})
+            if (*e) {
+                (void)hipGraphExecDestroy(*e);
+                *e = nullptr;
+            }
+    }
+
+    // Records body() on the context stream into an executable graph.
+    template <typename F>
+    pppm_status capture(hipGraphExec_t* exec, F body) {
+        PPPM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        pppm_status s = body();
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &gr);
+        if (s != PPPM_OK || e != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            (void)hipGetLastError();
+            if (s != PPPM_OK) return s;
+            PPPM_HIP_TRY(e);
+        }
+        e = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        PPPM_HIP_TRY(e);
+        return PPPM_OK;
+    }
+
+    size_t device_bytes() const { return ctx_bytes(p, vs.cap, es.cap); }
 
     pppm_status init(const pppm_params& prm) {
         p = prm;
@@ -215,6 +267,7 @@ struct pppm_ctx {
     }
 
     void release() {
+        drop_graphs();
         vs.release();
         es.release();
         dev_free(spos);
@@ -350,6 +403,7 @@ extern "C" pppm_status pppm_set_stream(pppm_ctx* c, void* s) {
     if (c->own_stream) (void)hipStreamDestroy(c->st);
     c->st = (hipStream_t)s;
     c->own_stream = false;
+    c->drop_graphs();
     return PPPM_OK;
 }
 
@@ -370,7 +424,45 @@ extern "C" pppm_status pppm_set_domain(pppm_ctx* c, const double origin[3], doub
     c->pg = make_geom(np);
     c->have_vort = false;
     c->have_far = false;
+    c->drop_graphs();
     return c->upload_dir();
+}
+
+extern "C" pppm_status pppm_update_params(pppm_ctx* c, const pppm_params* p) {
+    if (!c || !p) FAIL(PPPM_ERR_INVALID, "null arg");
+    TRY(validate_params(p));
+    if (p->g != c->p.g || p->precision != c->p.precision)
+        FAIL(PPPM_ERR_INVALID, "grid size and precision are fixed for a context");
+    PPPM_HIP_TRY(hipStreamSynchronize(c->st));
+    c->p = *p;
+    c->pg = make_geom(*p);
+    c->have_vort = false;
+    c->have_far = false;
+    c->drop_graphs();
+    return c->upload_dir();
+}
+
+extern "C" pppm_status pppm_get_device_bytes(const pppm_ctx* c, size_t* bytes) {
+    if (!c || !bytes) FAIL(PPPM_ERR_INVALID, "null arg");
+    *bytes = c->device_bytes();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_estimate_device_bytes(const pppm_params* p, uint32_t n, uint32_t m, int32_t other_end,
+                                                  size_t* bytes) {
+    TRY(validate_params(p));
+    if (!bytes) FAIL(PPPM_ERR_INVALID, "null arg");
+    (void)other_end;  // the segment-end buffer is allocated with the evaluation points
+    *bytes = ctx_bytes(*p, n ? std::max(n, 1024u) : 0, m ? std::max(m, 1024u) : 0);
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_set_graphs(pppm_ctx* c, int32_t on) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    PPPM_HIP_TRY(hipStreamSynchronize(c->st));
+    c->use_graphs = on != 0;
+    if (!c->use_graphs) c->drop_graphs();
+    return PPPM_OK;
 }
 
 extern "C" pppm_status pppm_get_params(const pppm_ctx* c, pppm_params* out) {
@@ -425,14 +517,24 @@ static pppm_status solve_mixed(pppm_ctx* c) {
     TRY(launch_dc_step(nullptr, nullptr, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn, st));
     TRY(read_rel(rel));
     int outer = 0, inner = 0;
-    while (outer < c->p.mg_max_cycles && rel[3] > c->p.mg_tol) {
+    // one outer step: e = V-cycle(A e = r32, e = 0); psi_next = psi + e;
+    // r32 = b - A psi_next and its FP64 norms
+    auto step = [c, g, outn, st](int cur) -> pppm_status {
         TRY(c->mg.fill(c->e32, 3 * c->G, st));
         TRY(c->mg.vcycle(c->e32, c->r32, 1.0f, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st));
+        return launch_dc_step(c->psi64[cur], c->e32, c->b64, c->psi64[1 - cur], c->r32, g, g, g, 3,
+                              c->mg.partials(), outn, st);
+    };
+    while (outer < c->p.mg_max_cycles && rel[3] > c->p.mg_tol) {
+        const int cur = c->psi_cur;
+        if (c->use_graphs) {
+            if (!c->g_mixed[cur]) TRY(c->capture(&c->g_mixed[cur], [&]() { return step(cur); }));
+            PPPM_HIP_TRY(hipGraphLaunch(c->g_mixed[cur], st));
+        } else {
+            TRY(step(cur));
+        }
         inner++;
-        int nxt = 1 - c->psi_cur;
-        TRY(launch_dc_step(c->psi64[c->psi_cur], c->e32, c->b64, c->psi64[nxt], c->r32, g, g, g, 3,
-                           c->mg.partials(), outn, st));
-        c->psi_cur = nxt;
+        c->psi_cur = 1 - cur;
         outer++;
         TRY(read_rel(rel));
     }
@@ -474,12 +576,39 @@ extern "C" pppm_status pppm_compute_far_field(pppm_ctx* c) {
             c->stats.mg_cycles = 1;
             c->stats.mg_converged = 1;
         } else {
+            // V-cycles until the FP64-accumulated relative residual meets the
+            // tolerance or stops improving (FP32 rounding floor)
+            double rn[3], bn[3], w = 0, prev;
+            auto worst = [&]() {
+                double m = 0;
+                for (int k = 0; k < 3; k++) {
+                    c->stats.mg_relres[k] = bn[k] > 0 ? rn[k] / bn[k] : rn[k];
+                    if (c->stats.mg_relres[k] > m) m = c->stats.mg_relres[k];
+                }
+                return m;
+            };
+            TRY(c->mg.norms(c->psi32, c->wg, bscale, rn, bn, st));
+            w = worst();
             int cyc = 0;
-            pppm_status s = c->mg.solve(c->psi32, c->wg, bscale, c->p.nu1, c->p.nu2, (float)c->p.mg_omega,
-                                        c->p.mg_tol, c->p.mg_max_cycles, &cyc, c->stats.mg_relres, st);
-            if (s != PPPM_OK && s != PPPM_ERR_NOT_CONVERGED) return s;
+            auto cycle = [c, bscale, st]() -> pppm_status {
+                TRY(c->mg.vcycle(c->psi32, c->wg, bscale, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st));
+                return c->mg.norms_launch(c->psi32, c->wg, bscale, st);
+            };
+            while (cyc < c->p.mg_max_cycles && w > c->p.mg_tol) {
+                if (c->use_graphs) {
+                    if (!c->g_plain) TRY(c->capture(&c->g_plain, cycle));
+                    PPPM_HIP_TRY(hipGraphLaunch(c->g_plain, st));
+                } else {
+                    TRY(cycle());
+                }
+                cyc++;
+                TRY(c->mg.norms_read(rn, bn, st));
+                prev = w;
+                w = worst();
+                if (cyc >= 3 && w > 0.9 * prev) break;
+            }
             c->stats.mg_cycles = cyc;
-            c->stats.mg_converged = s == PPPM_OK;
+            c->stats.mg_converged = w <= c->p.mg_tol;
         }
         c->stats.mg_outer = 0;
         psi = c->psi32;

@@ -8,6 +8,7 @@
 #include "mg.hpp"
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace pppm {
 
@@ -56,6 +57,288 @@ __global__ void __launch_bounds__(256) k_rbgs(T* __restrict__ x, const T* __rest
     else
         xn = ((T)1 - omega) * xc[t] + omega * r / ((T)6 - beta * (T)nout);
     xc[t] = xn;
+}
+
+// Tile of the blocked smoother k_rb2 (below): 64 x 16 cells per workgroup.
+constexpr int kTbTX = 64;
+constexpr int kTbTY = 16;
+
+struct TbArgs {
+    const float* xin;    // level array before the pass (read only)
+    float* x;            // level array after the pass (a different buffer)
+    const float* b;      // right-hand side (times bscale)
+    float bscale;
+    int nx, ny, nz;
+    size_t G;            // cells per component
+    float omega, beta;   // SOR factor, ghost factor of this level
+    int zc;              // z planes per workgroup chunk (even)
+    // RESTRICT: residual restricted into bn (coarse dims cx, cy, cz = n / 2)
+    float* bn;
+    int cx, cy, cz;
+    size_t Gc;
+    // PROLONG: x += trilinear(xc) on load (coarse dims cx, cy, cz, fine dims
+    // even), ghost betac
+    const float* xc;
+    float betac;
+};
+
+// Trilinear prolongation (k_prolong's weights) at fine cell (i, j, k) from a
+// coarse window held in LDS: cw[slot(c)][cy - cy0][cx - cx0] for coarse planes
+// c, rows cy and columns cx; a neighbour outside the coarse grid is betac times
+// the parent value.  Fine dimensions are even here, so parents are i / 2.
+template <int CLX, int CLY>
+__device__ inline float tb_prolong_lds(const float (*cw)[CLY][CLX], int i, int j, int k, int cx0, int cy0, int cx,
+                                       int cy, int cz, float betac) {
+    const int pi = i >> 1, pj = j >> 1, pk = k >> 1;
+    int ni = (i & 1) ? pi + 1 : pi - 1, nj = (j & 1) ? pj + 1 : pj - 1, nk = (k & 1) ? pk + 1 : pk - 1;
+    float fi = 1.f, fj = 1.f, fk = 1.f;
+    if (ni < 0 || ni > cx - 1) { ni = pi; fi = betac; }
+    if (nj < 0 || nj > cy - 1) { nj = pj; fj = betac; }
+    if (nk < 0 || nk > cz - 1) { nk = pk; fk = betac; }
+    const int sp = pk % 3, sn = nk % 3;
+    const int ip = pi - cx0, in_ = ni - cx0, jp = pj - cy0, jn = nj - cy0;
+    const float p0 = 0.75f * cw[sp][jp][ip] + 0.25f * fi * cw[sp][jp][in_];
+    const float p1 = 0.75f * cw[sp][jn][ip] + 0.25f * fi * cw[sp][jn][in_];
+    const float p2 = 0.75f * cw[sn][jp][ip] + 0.25f * fi * cw[sn][jp][in_];
+    const float p3 = 0.75f * cw[sn][jn][ip] + 0.25f * fi * cw[sn][jn][in_];
+    const float q0 = 0.75f * p0 + 0.25f * fj * p1;
+    const float q1 = 0.75f * p2 + 0.25f * fj * p3;
+    return 0.75f * q0 + 0.25f * fk * q1;
+}
+
+// Blocked red-black smoother: one red-black iteration (two half-sweeps) per
+// pass over the grid.  A workgroup owns a 64 x 16 tile plus an H-cell halo in
+// x and y and a chunk of z planes plus an H-plane halo; it streams the planes
+// through LDS rings and applies half-sweep t to plane k - t right after plane
+// k arrives.  In place in LDS is exact for red-black ordering: half-sweep t
+// reads only the colour written by half-sweep t - 1, and in step k plane
+// k - t has received half-sweeps 1..t-1 while its neighbours have not yet
+// received t + 1.  Halo cells are recomputed redundantly; a cell at distance
+// d from the loaded region's edge is valid for half-sweeps t <= d (sides on
+// the grid boundary use the exact ghost rule).  H = 2, or 3 when the pass
+// also forms the residual (which needs final values one cell beyond the
+// tile).  Cells outside the grid hold zero in LDS and are never updated, and
+// each LDS plane has one spare zero column and row, so neighbour reads need
+// no bounds tests.  The pass reads xin and writes x: halos are loaded by other
+// workgroups while this one writes its tile, so the two must not alias.
+// BETA: coarse level, out-of-grid neighbours are beta times the cell value.
+template <bool RESTRICT, bool PROLONG, bool XZERO, bool BETA>
+__global__ void __launch_bounds__(256) k_rb2(TbArgs a) {
+    constexpr int H = RESTRICT ? 3 : 2;
+    constexpr int LX = kTbTX + 2 * H, LY = kTbTY + 2 * H;
+    constexpr int LXP = LX + 1, LYP = LY + 1, PL = LXP * LYP;  // padded plane
+    constexpr int NR = RESTRICT ? 5 : 4;  // planes of x alive at once
+    constexpr int NB = RESTRICT ? 4 : 3;  // planes of b alive at once
+    constexpr int NL = (LX * LY + 255) / 256;
+    constexpr int HX = LX / 2;                    // cell pairs per row
+    constexpr int NP = (HX * LY + 255) / 256;     // pair slots per thread
+    constexpr int CLX = PROLONG ? LX / 2 + 3 : 1, CLY = PROLONG ? LY / 2 + 3 : 1;
+    constexpr int NC = (CLX * CLY + 255) / 256;
+    __shared__ float xs[NR * PL];
+    __shared__ float bs[NB * PL];
+    __shared__ float cw[PROLONG ? 3 : 1][CLY][CLX];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int gx0 = blockIdx.x * kTbTX - H, gy0 = blockIdx.y * kTbTY - H;
+    const int nzc = (nz + a.zc - 1) / a.zc;
+    const int comp = blockIdx.z / nzc, zi = blockIdx.z - comp * nzc;
+    const int z0 = zi * a.zc, z1 = min(z0 + a.zc, nz);
+    const float* xsrc = a.xin + (size_t)comp * a.G;
+    float* xc = a.x + (size_t)comp * a.G;
+    const float* bcp = a.b + (size_t)comp * a.G;
+    const float* xcoarse = PROLONG ? a.xc + (size_t)comp * a.Gc : nullptr;
+    const int cx0 = (gx0 >> 1) - 1, cy0 = (gy0 >> 1) - 1;  // coarse window origin
+    const int klo = max(z0 - H, 0), khi = min(z1 + H, nz);
+    const bool xl_ghost = gx0 <= 0, xr_ghost = gx0 + LX >= nx;
+    const bool yl_ghost = gy0 <= 0, yr_ghost = gy0 + LY >= ny;
+    const float omega = a.omega, beta = a.beta;
+    const int tid = threadIdx.x;
+
+    // zero everything once: spare column / row and out-of-grid cells stay 0
+    for (int q = tid; q < NR * PL; q += 256) xs[q] = 0.f;
+    for (int q = tid; q < NB * PL; q += 256) bs[q] = 0.f;
+
+    // this thread's pair slots: row ly, pair lx2 (cells 2 lx2 and 2 lx2 + 1)
+    int sbase[NP], sly[NP], slx2[NP], spar[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+        const int q = tid + 256 * u;
+        const int ly = q < HX * LY ? q / HX : LY + 8;  // out of every update range
+        const int lx2 = q < HX * LY ? q - ly * HX : 0;
+        sly[u] = ly;
+        slx2[u] = lx2;
+        sbase[u] = ly * LXP + 2 * lx2;
+        spar[u] = (gx0 + gy0 + ly) & 1;
+    }
+
+    // Plane k + 1 is fetched into registers while plane k is swept, so the
+    // global loads overlap the sweeps; the coarse planes the prolongation
+    // needs go through a three-plane LDS ring the same way.
+    float px[NL], pb[NL], pc[NC];
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int q = tid + 256 * u;
+            const int ly = q / LX, lx = q - ly * LX;
+            const int i = gx0 + lx, j = gy0 + ly;
+            float xv = 0.f, bv = 0.f;
+            if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) {
+                const size_t t = ((size_t)k * ny + j) * nx + i;
+                if (!XZERO) xv = xsrc[t];
+                bv = bcp[t];
+            }
+            px[u] = xv;
+            pb[u] = bv;
+        }
+    };
+    auto fetch_coarse = [&](int c) {
+#pragma unroll
+        for (int u = 0; u < NC; u++) {
+            const int q = tid + 256 * u;
+            const int cyl = q / CLX, cxl = q - cyl * CLX;
+            const int ci = cx0 + cxl, cj = cy0 + cyl;
+            float v = 0.f;
+            if (q < CLX * CLY && c >= 0 && c < a.cz && ci >= 0 && ci < a.cx && cj >= 0 && cj < a.cy)
+                v = xcoarse[((size_t)c * a.cy + cj) * a.cx + ci];
+            pc[u] = v;
+        }
+    };
+    auto store_coarse = [&](int c) {
+        if (c < 0 || c >= a.cz) return;
+#pragma unroll
+        for (int u = 0; u < NC; u++) {
+            const int q = tid + 256 * u;
+            if (q < CLX * CLY) cw[c % 3][q / CLX][q % CLX] = pc[u];
+        }
+    };
+    if (PROLONG) {
+        // coarse planes of the first fine plane, synchronously
+        for (int c = (klo >> 1) - 1; c <= (klo >> 1) + 1; c++) {
+            fetch_coarse(c);
+            store_coarse(c);
+        }
+    }
+    fetch(klo);
+    __syncthreads();  // zeroed rings before the first stores
+    float rsum = 0.f;  // RESTRICT: this thread's coarse cell, partial over a plane pair
+    for (int k = klo; k < khi + 2 + (RESTRICT ? 1 : 0); k++) {
+        if (PROLONG) {
+            if (k < khi && k > klo && (k & 1) == 0) store_coarse((k >> 1) + 1);  // fetched during plane k - 1
+            __syncthreads();
+        }
+        if (k < khi) {
+            float* xr = xs + (k % NR) * PL;
+            float* br = bs + (k % NB) * PL;
+#pragma unroll
+            for (int u = 0; u < NL; u++) {
+                const int q = tid + 256 * u;
+                if (q < LX * LY) {
+                    const int ly = q / LX, lx = q - ly * LX;
+                    float xv = px[u];
+                    if (PROLONG) {
+                        const int i = gx0 + lx, j = gy0 + ly;
+                        if (i >= 0 && i < nx && j >= 0 && j < ny)
+                            xv += tb_prolong_lds<CLX, CLY>(cw, i, j, k, cx0, cy0, a.cx, a.cy, a.cz, a.betac);
+                    }
+                    xr[ly * LXP + lx] = xv;
+                    br[ly * LXP + lx] = a.bscale * pb[u];
+                }
+            }
+        }
+        __syncthreads();
+        if (k + 1 < khi) {
+            fetch(k + 1);
+            if (PROLONG && ((k + 1) & 1) == 0) fetch_coarse(((k + 1) >> 1) + 1);
+        }
+#pragma unroll
+        for (int t = 1; t <= 2; t++) {
+            const int p = k - t;
+            const int plo = klo > 0 ? klo + t : 0;
+            const int phi = khi < nz ? khi - 1 - t : nz - 1;
+            if (p >= plo && p <= phi) {
+                const int col = t & 1;  // odd colour first, as the reference
+                const int ax = xl_ghost ? -gx0 : t, bxl = xr_ghost ? nx - 1 - gx0 : LX - 1 - t;
+                const int ay = yl_ghost ? -gy0 : t, byl = yr_ghost ? ny - 1 - gy0 : LY - 1 - t;
+                const float* xm = xs + ((p + NR - 1) % NR) * PL;
+                float* x0 = xs + (p % NR) * PL;
+                const float* xp = xs + ((p + 1) % NR) * PL;
+                const float* b0 = bs + (p % NB) * PL;
+                const bool zlo = p == 0, zhi = p == nz - 1;
+                const int pc_ = (col + p) & 1;
+#pragma unroll
+                for (int u = 0; u < NP; u++) {
+                    const int bit = spar[u] ^ pc_;
+                    const int lx = 2 * slx2[u] + bit, ly = sly[u];
+                    if (lx < ax || lx > bxl || ly < ay || ly > byl) continue;
+                    const int o = sbase[u] + bit;
+                    float r = x0[o - 1] + x0[o + 1];
+                    r += x0[o - LXP] + x0[o + LXP];
+                    r += (zlo ? 0.f : xm[o]) + (zhi ? 0.f : xp[o]);
+                    r -= b0[o];
+                    const float xo = x0[o];
+                    float d = 1.0f / 6.0f;
+                    if (BETA) {
+                        const int i = gx0 + lx, j = gy0 + ly;
+                        const int nout = (i == 0) + (i == nx - 1) + (j == 0) + (j == ny - 1) + zlo + zhi;
+                        d = 1.0f / (6.0f - beta * (float)nout);
+                    }
+                    x0[o] = xo + omega * (r * d - xo);
+                }
+            }
+            __syncthreads();
+        }
+        // plane k - 2 is final: write its tile interior
+        const int pw = k - 2;
+        if (pw >= z0 && pw < z1 && pw < khi) {
+            const float* xr = xs + (pw % NR) * PL;
+            for (int q = tid; q < kTbTX * kTbTY; q += 256) {
+                const int ly = H + q / kTbTX, lx = H + (q % kTbTX);
+                const int i = gx0 + lx, j = gy0 + ly;
+                if (i < nx && j < ny) xc[((size_t)pw * ny + j) * nx + i] = xr[ly * LXP + lx];
+            }
+        }
+        if (RESTRICT) {
+            // residual of plane pr = k - 3 (its neighbours are final), summed
+            // into the coarse cell (I, J) owned by this thread: 32 x 8 per tile
+            const int pr = k - 3;
+            if (pr >= z0 && pr < z1) {
+                const int I = tid % (kTbTX / 2), J = tid / (kTbTX / 2);
+                const float* xm = xs + ((pr + NR - 1) % NR) * PL;
+                const float* x0 = xs + (pr % NR) * PL;
+                const float* xp = xs + ((pr + 1) % NR) * PL;
+                const float* b0 = bs + (pr % NB) * PL;
+                const bool zlo = pr == 0, zhi = pr == nz - 1;
+                float part = 0.f;
+#pragma unroll
+                for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+                    for (int di = 0; di < 2; di++) {
+                        const int lx = H + 2 * I + di, ly = H + 2 * J + dj;
+                        const int i = gx0 + lx, j = gy0 + ly;
+                        if (i >= nx || j >= ny) continue;
+                        const int o = ly * LXP + lx;
+                        const float xo = x0[o];
+                        float sres = x0[o - 1] + x0[o + 1];
+                        sres += x0[o - LXP] + x0[o + LXP];
+                        sres += (zlo ? 0.f : xm[o]) + (zhi ? 0.f : xp[o]);
+                        sres -= 6.0f * xo;
+                        if (BETA) {
+                            const int nout = (i == 0) + (i == nx - 1) + (j == 0) + (j == ny - 1) + zlo + zhi;
+                            sres += beta * (float)nout * xo;
+                        }
+                        part += b0[o] - sres;
+                    }
+                if ((pr & 1) == 0) {
+                    rsum = part;
+                } else {
+                    const int gI = blockIdx.x * (kTbTX / 2) + I, gJ = blockIdx.y * (kTbTY / 2) + J, gK = pr / 2;
+                    if (gI < a.cx && gJ < a.cy && gK < a.cz)
+                        a.bn[(size_t)comp * a.Gc + ((size_t)gK * a.cy + gJ) * a.cx + gI] = (rsum + part) * 0.5f;
+                }
+            }
+            // the next load reuses the ring slots of planes pr - 1 (x) and pr (b)
+            __syncthreads();
+        }
+    }
 }
 
 template <typename T>
@@ -215,12 +498,18 @@ __global__ void __launch_bounds__(256) k_resnorm(const T* __restrict__ x, const 
     const T* xc = x + (size_t)comp * G;
     const T* bc = b + (size_t)comp * G;
     double rr = 0, bb = 0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < G; t += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
-        T r = residual_at(xc, bc, bscale, i, j, k, nx, ny, nz, (T)0);
-        double bv = (double)bscale * (double)bc[t];
-        rr += (double)r * (double)r;
-        bb += bv * bv;
+    // block b takes a contiguous run of x rows; lanes run along x
+    const int rows = ny * nz, per = (rows + gridDim.x - 1) / gridDim.x;
+    const int r0 = blockIdx.x * per, r1 = min(r0 + per, rows);
+    for (int row = r0; row < r1; row++) {
+        const int j = row % ny, k = row / ny;
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const size_t t = (size_t)row * nx + i;
+            T r = residual_at(xc, bc, bscale, i, j, k, nx, ny, nz, (T)0);
+            double bv = (double)bscale * (double)bc[t];
+            rr += (double)r * (double)r;
+            bb += bv * bv;
+        }
     }
     double s1 = block_sum_256(rr, lds);
     double s2 = block_sum_256(bb, lds);
@@ -258,27 +547,33 @@ __global__ void __launch_bounds__(256) k_dc_step(const double* __restrict__ xin,
     const double* xi = xin ? xin + (size_t)comp * G : nullptr;
     const float* ec = e ? e + (size_t)comp * G : nullptr;
     double rr = 0, bb = 0;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < G; t += (size_t)gridDim.x * blockDim.x) {
-        int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
-        auto X = [&](size_t q) { return (xi ? xi[q] : 0.0) + (ec ? (double)ec[q] : 0.0); };
-        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-        double xc = X(t);
-        double lv = i > 0 ? X(t - 1) : 0.0;
-        double rv = i < nx - 1 ? X(t + 1) : 0.0;
-        double uv = j < ny - 1 ? X(t + sy) : 0.0;
-        double dv = j > 0 ? X(t - sy) : 0.0;
-        double fv = k > 0 ? X(t - sz) : 0.0;
-        double bv = k < nz - 1 ? X(t + sz) : 0.0;
-        double s = lv + rv;
-        s += uv + dv;
-        s += fv + bv;
-        s -= 6.0 * xc;
-        double bval = b64[(size_t)comp * G + t];
-        double r = bval - s;
-        xout[(size_t)comp * G + t] = xc;
-        r32[(size_t)comp * G + t] = (float)r;
-        rr += r * r;
-        bb += bval * bval;
+    auto X = [&](size_t q) { return (xi ? xi[q] : 0.0) + (ec ? (double)ec[q] : 0.0); };
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    // block b takes a contiguous run of x rows; lanes run along x
+    const int rows = ny * nz, per = (rows + gridDim.x - 1) / gridDim.x;
+    const int r0 = blockIdx.x * per, r1 = min(r0 + per, rows);
+    for (int row = r0; row < r1; row++) {
+        const int j = row % ny, k = row / ny;
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const size_t t = (size_t)row * nx + i;
+            double xc = X(t);
+            double lv = i > 0 ? X(t - 1) : 0.0;
+            double rv = i < nx - 1 ? X(t + 1) : 0.0;
+            double uv = j < ny - 1 ? X(t + sy) : 0.0;
+            double dv = j > 0 ? X(t - sy) : 0.0;
+            double fv = k > 0 ? X(t - sz) : 0.0;
+            double bv = k < nz - 1 ? X(t + sz) : 0.0;
+            double s = lv + rv;
+            s += uv + dv;
+            s += fv + bv;
+            s -= 6.0 * xc;
+            double bval = b64[(size_t)comp * G + t];
+            double r = bval - s;
+            xout[(size_t)comp * G + t] = xc;
+            r32[(size_t)comp * G + t] = (float)r;
+            rr += r * r;
+            bb += bval * bval;
+        }
     }
     double s1 = block_sum_256(rr, lds);
     double s2 = block_sum_256(bb, lds);
@@ -378,6 +673,26 @@ std::vector<double> coarse_tables(int nx, int ny, int nz, double beta) {
 }
 
 template <typename T>
+size_t MGHier<T>::bytes_for(int nx, int ny, int nz, int ncomp) {
+    size_t b = 0, cells = (size_t)nx * ny * nz;
+    int x = nx, y = ny, z = nz, L = 1;
+    if (std::is_same<T, float>::value) b += cells * ncomp * sizeof(T);
+    while ((long)x * y * z > 512 && L < 16) {
+        x = x / 2 > 0 ? x / 2 : 1;
+        y = y / 2 > 0 ? y / 2 : 1;
+        z = z / 2 > 0 ? z / 2 : 1;
+        L++;
+        cells = (size_t)x * y * z;
+        b += 2 * cells * ncomp * sizeof(T);
+        if (std::is_same<T, float>::value) b += cells * ncomp * sizeof(T);
+    }
+    // coarse tables (two bases, <= 3 * 512^2 doubles each at most) and norm partials
+    b += 2 * (size_t)(x * x + y * y + z * z + x + y + z) * sizeof(double);
+    b += (size_t)kNormBlocks * 3 * 2 * sizeof(double) + 6 * sizeof(double);
+    return b;
+}
+
+template <typename T>
 pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
     release();
     if (nx < 1 || ny < 1 || nz < 1 || ncomp < 1 || ncomp > 3) return PPPM_ERR_INVALID;
@@ -400,7 +715,19 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
     L_ += 1;
     x_.assign(L_, nullptr);
     b_.assign(L_, nullptr);
+    t_.assign(L_, nullptr);
     bytes_ = 0;
+    if (std::is_same<T, float>::value) {
+        // out-of-place targets of the blocked smoother, every level
+        for (int l = 0; l < L_; l++) {
+            size_t n = cells(l) * ncomp_;
+            if (hipMalloc(&t_[l], n * sizeof(T)) != hipSuccess) {
+                release();
+                return PPPM_ERR_NOMEM;
+            }
+            bytes_ += n * sizeof(T);
+        }
+    }
     for (int l = 1; l < L_; l++) {
         size_t n = cells(l) * ncomp_;
         if (hipMalloc(&x_[l], n * sizeof(T)) != hipSuccess || hipMalloc(&b_[l], n * sizeof(T)) != hipSuccess) {
@@ -419,6 +746,8 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
         release();
         return PPPM_ERR_NOMEM;
     }
+    bytes_ += (tc.size() + tr.size()) * sizeof(double) + (size_t)kNormBlocks * 3 * 2 * sizeof(double) +
+              6 * sizeof(double);
     PPPM_HIP_TRY(hipMemcpy(tab_cons_, tc.data(), tc.size() * sizeof(double), hipMemcpyHostToDevice));
     PPPM_HIP_TRY(hipMemcpy(tab_ref_, tr.data(), tr.size() * sizeof(double), hipMemcpyHostToDevice));
     return PPPM_OK;
@@ -430,8 +759,11 @@ void MGHier<T>::release() {
         if (p) (void)hipFree(p);
     for (T* p : b_)
         if (p) (void)hipFree(p);
+    for (T* p : t_)
+        if (p) (void)hipFree(p);
     x_.clear();
     b_.clear();
+    t_.clear();
     if (tab_cons_) (void)hipFree(tab_cons_);
     if (tab_ref_) (void)hipFree(tab_ref_);
     if (part_) (void)hipFree(part_);
@@ -451,8 +783,76 @@ pppm_status MGHier<T>::fill(T* p, size_t n, hipStream_t st) {
 }
 
 template <typename T>
+__global__ void k_copy(T* __restrict__ d, const T* __restrict__ s, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) d[i] = s[i];
+}
+
+template <typename T>
+pppm_status MGHier<T>::copy(T* d, const T* s, size_t n, hipStream_t st) {
+    if (n == 0) return PPPM_OK;
+    size_t blocks = div_up64(n, 256);
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_copy<T>, dim3((uint32_t)blocks), dim3(256), 0, st, d, s, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+static int tb_zchunk(int nz) { return nz >= 128 ? 32 : (nz >= 32 ? 16 : nz + (nz & 1)); }
+
+// One launch of k_rb2 over all components.
+template <bool R, bool P, bool Z>
+static pppm_status launch_rb2(const TbArgs& a, int ncomp, hipStream_t st) {
+    const int nzc = (a.nz + a.zc - 1) / a.zc;
+    dim3 grid(div_up(a.nx, kTbTX), div_up(a.ny, kTbTY), nzc * ncomp), blk(256);
+    if (a.beta != 0.f)
+        hipLaunchKernelGGL((k_rb2<R, P, Z, true>), grid, blk, 0, st, a);
+    else
+        hipLaunchKernelGGL((k_rb2<R, P, Z, false>), grid, blk, 0, st, a);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+static pppm_status smooth_tb(const T*, T*, const T*, T, int, int, int, int, size_t, T, T, hipStream_t) {
+    return PPPM_ERR_INVALID;
+}
+// one red-black iteration, xin -> x
+template <>
+pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float bscale, int nx, int ny, int nz,
+                             int ncomp, size_t G, float omega, float beta, hipStream_t st) {
+    TbArgs a{};
+    a.xin = xin;
+    a.x = x;
+    a.b = b;
+    a.bscale = bscale;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.G = G;
+    a.omega = omega;
+    a.beta = beta;
+    a.zc = tb_zchunk(nz);
+    return launch_rb2<false, false, false>(a, ncomp, st);
+}
+
+template <typename T>
 pppm_status MGHier<T>::smooth(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st) {
     const int nx = d_[l][0], ny = d_[l][1], nz = d_[l][2];
+    if (std::is_same<T, float>::value && tb_ && iters > 0) {
+        // one out-of-place pass per iteration, x -> t -> x ...; an odd count
+        // ends in t and is copied back
+        T* tmp = t_[l];
+        for (int q = 0; q < iters; q++) {
+            const T* src = (q & 1) ? tmp : x;
+            T* dst = (q & 1) ? x : tmp;
+            pppm_status s = smooth_tb<T>(src, dst, b, bscale, nx, ny, nz, ncomp_, cells(l), omega, beta, st);
+            if (s != PPPM_OK) return s;
+        }
+        if (iters & 1) return copy(x, tmp, cells(l) * ncomp_, st);
+        return PPPM_OK;
+    }
     dim3 blk(64, 4), grid(div_up(div_up(nx, 2), 64), div_up(ny, 4), nz * ncomp_);
     for (int it = 0; it < iters; it++)
         for (int color = 1; color >= 0; color--) {
@@ -522,9 +922,89 @@ static pppm_status vcycle_from(MGHier<T>& m, std::vector<T*>& X, std::vector<T*>
     return PPPM_OK;
 }
 
+// Fused V-cycle (float, consistent transfers): per level nu1 blocked passes
+// of one red-black iteration each, the last also forming the residual and its
+// restriction, and nu2 passes on the way up, the first adding the prolonged
+// coarse correction as it loads x.
+template <typename T>
+static pppm_status vcycle_fused(MGHier<T>&, std::vector<T*>&, std::vector<T*>&, T, int, int, T, hipStream_t) {
+    return PPPM_ERR_INVALID;
+}
+template <>
+pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::vector<float*>& B, float bscale0,
+                                int nu1, int nu2, float omega, hipStream_t st) {
+    const int L = m.levels(), nc = m.ncomp();
+    pppm_status s;
+    auto level_args = [&](int l) {
+        const int* d = m.dims(l);
+        const int* c = m.dims(l + 1);
+        TbArgs a{};
+        a.b = B[l];
+        a.bscale = l == 0 ? bscale0 : 1.0f;
+        a.nx = d[0];
+        a.ny = d[1];
+        a.nz = d[2];
+        a.G = m.cells(l);
+        a.omega = omega;
+        a.beta = (float)mg_beta(l);
+        a.zc = tb_zchunk(d[2]);
+        a.cx = c[0];
+        a.cy = c[1];
+        a.cz = c[2];
+        a.Gc = m.cells(l + 1);
+        return a;
+    };
+    // down: nu1 passes per level, x from X (or zero below the finest level)
+    // ping-ponging with the level's temporary; the last pass also restricts
+    std::vector<float*> cur(L, nullptr);
+    for (int l = 0; l < L - 1; l++) {
+        float* src = X[l];
+        float* dst = m.temp(l);
+        for (int q = 0; q < nu1; q++) {
+            TbArgs a = level_args(l);
+            a.xin = src;
+            a.x = dst;
+            a.bn = B[l + 1];
+            const bool zero = l > 0 && q == 0, last = q == nu1 - 1;
+            if (last) s = zero ? launch_rb2<true, false, true>(a, nc, st) : launch_rb2<true, false, false>(a, nc, st);
+            else s = zero ? launch_rb2<false, false, true>(a, nc, st) : launch_rb2<false, false, false>(a, nc, st);
+            if (s != PPPM_OK) return s;
+            std::swap(src, dst);
+        }
+        cur[l] = src;
+    }
+    if ((s = m.coarse(X[L - 1], B[L - 1], L == 1 ? bscale0 : 1.0f, true, st)) != PPPM_OK) return s;
+    // up: nu2 passes per level, the first adds the prolonged correction on load
+    for (int l = L - 2; l >= 0; l--) {
+        float* src = cur[l];
+        float* dst = src == X[l] ? m.temp(l) : X[l];
+        for (int q = 0; q < nu2; q++) {
+            TbArgs a = level_args(l);
+            a.xin = src;
+            a.x = dst;
+            a.xc = X[l + 1];
+            a.betac = (float)mg_beta(l + 1);
+            s = q == 0 ? launch_rb2<false, true, false>(a, nc, st) : launch_rb2<false, false, false>(a, nc, st);
+            if (s != PPPM_OK) return s;
+            std::swap(src, dst);
+        }
+        if (src != X[l] && (s = m.copy(X[l], src, m.cells(l) * nc, st)) != PPPM_OK) return s;
+    }
+    return PPPM_OK;
+}
+
 template <typename T>
 pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent,
                               hipStream_t st) {
+    bool even = true;  // the fused prolongation takes parents as i / 2
+    for (int l = 0; l + 1 < L_; l++) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
+    if (std::is_same<T, float>::value && consistent && tb_ && (nu1 == 1 || nu1 == 2) && (nu2 == 1 || nu2 == 2) &&
+        L_ > 1 && even) {
+        std::vector<T*> X = x_, B = b_;
+        X[0] = x0;
+        B[0] = const_cast<T*>(b0);
+        return vcycle_fused<T>(*this, X, B, bscale0, nu1, nu2, omega, st);
+    }
     std::vector<T*> X = x_, B = b_;
     X[0] = x0;
     B[0] = const_cast<T*>(b0);
@@ -552,7 +1032,7 @@ pppm_status MGHier<T>::ref_fmg(T* x0, const T* b0, T bscale0, int nu1, int nu2, 
 }
 
 template <typename T>
-pppm_status MGHier<T>::norms(const T* x0, const T* b0, T bscale0, double* rn, double* bn, hipStream_t st) {
+pppm_status MGHier<T>::norms_launch(const T* x0, const T* b0, T bscale0, hipStream_t st) {
     const int nx = d_[0][0], ny = d_[0][1], nz = d_[0][2];
     double* out = part_ + (size_t)kNormBlocks * 3 * 2;
     hipLaunchKernelGGL(k_resnorm<T>, dim3(kNormBlocks, ncomp_), dim3(256), 0, st, x0, b0, bscale0, nx, ny, nz,
@@ -560,6 +1040,12 @@ pppm_status MGHier<T>::norms(const T* x0, const T* b0, T bscale0, double* rn, do
     PPPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp_), dim3(256), 0, st, part_, kNormBlocks, out);
     PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::norms_read(double* rn, double* bn, hipStream_t st) {
+    double* out = part_ + (size_t)kNormBlocks * 3 * 2;
     PPPM_HIP_TRY(hipMemcpyAsync(hnorm_, out, 2 * ncomp_ * sizeof(double), hipMemcpyDeviceToHost, st));
     PPPM_HIP_TRY(hipStreamSynchronize(st));
     for (int c = 0; c < ncomp_; c++) {
@@ -567,6 +1053,13 @@ pppm_status MGHier<T>::norms(const T* x0, const T* b0, T bscale0, double* rn, do
         bn[c] = std::sqrt(hnorm_[2 * c + 1]);
     }
     return PPPM_OK;
+}
+
+template <typename T>
+pppm_status MGHier<T>::norms(const T* x0, const T* b0, T bscale0, double* rn, double* bn, hipStream_t st) {
+    pppm_status s = norms_launch(x0, b0, bscale0, st);
+    if (s != PPPM_OK) return s;
+    return norms_read(rn, bn, st);
 }
 
 template <typename T>

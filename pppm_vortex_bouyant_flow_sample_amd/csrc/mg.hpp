@@ -29,6 +29,8 @@ class MGHier {
     const int* dims(int l) const { return d_[l]; }
     size_t cells(int l) const { return (size_t)d_[l][0] * d_[l][1] * d_[l][2]; }
     size_t device_bytes() const { return bytes_; }
+    // device bytes init(nx, ny, nz, ncomp) allocates
+    static size_t bytes_for(int nx, int ny, int nz, int ncomp);
 
     // One V(nu1,nu2) cycle from level l0 on (x0, b0 * bscale0) at level 0.
     // consistent: trilinear prolongation and boundary-consistent coarse
@@ -39,6 +41,10 @@ class MGHier {
     pppm_status ref_fmg(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, hipStream_t st);
     // ||b*bscale - A x||_2 and ||b*bscale||_2 per component (FP64 sums), synchronous.
     pppm_status norms(const T* x0, const T* b0, T bscale0, double* rn, double* bn, hipStream_t st);
+    // the same in two halves: device reduction only (capturable), then the
+    // synchronous readback of the reduced sums
+    pppm_status norms_launch(const T* x0, const T* b0, T bscale0, hipStream_t st);
+    pppm_status norms_read(double* rn, double* bn, hipStream_t st);
     // Repeated V-cycles until max relative residual <= tol or max_cycles.
     pppm_status solve(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, double tol, int max_cycles,
                       int* cycles, double* relres /*[ncomp]*/, hipStream_t st);
@@ -49,19 +55,27 @@ class MGHier {
     pppm_status prolong(int l, T* x, const T* xc, int mode, T betac, hipStream_t st);
     pppm_status coarse(T* x, const T* b, T bscale, bool consistent, hipStream_t st);
     pppm_status fill(T* p, size_t n, hipStream_t st);
+    pppm_status copy(T* d, const T* s, size_t n, hipStream_t st);
+    // out-of-place target of the blocked smoother at level l (float only)
+    T* temp(int l) { return t_[l]; }
     double* partials() { return part_; }
     double* host_norms() { return hnorm_; }
     static constexpr int kNormBlocks = 512;
+    // use the temporally blocked smoother (float only)
+    void set_blocked_smoother(bool on) { tb_ = on; }
+    bool blocked_smoother() const { return tb_; }
 
    private:
     int L_ = 0, ncomp_ = 1;
     int d_[16][3];
     std::vector<T*> x_, b_;       // levels >= 1
+    std::vector<T*> t_;           // all levels: blocked-smoother ping-pong (float)
     double* tab_cons_ = nullptr;  // coarse eigenbasis, consistent ghosts
     double* tab_ref_ = nullptr;   // coarse eigenbasis, zero ghosts
     double* part_ = nullptr;      // norm partial sums
     double* hnorm_ = nullptr;     // pinned host copy of the reduced norms
     size_t bytes_ = 0;
+    bool tb_ = true;
 };
 
 // FP64 defect-correction step of the mixed-precision solve:

@@ -2,8 +2,6 @@
 // Biot-Savart correction and the unsort to the caller's order
 // (GPU_kernel.cu:2886-2964 BiotSavartInterpolate_far_field_kernel2,
 //  2262-2436 BiotSavartPPCorrScaleMN_kernel, BiotSavartSolverImp.cpp:452-484).
-// One lane per sorted evaluation point; the vortices of each row of hash cells
-// (fixed y, z; x from i-K-1 to i+K+1) are one contiguous sorted segment.
 #include "pppm_internal.hpp"
 
 namespace pppm {
@@ -46,6 +44,12 @@ __device__ inline float coeff_gaussian(float r2, const NearConst& nc) {
     return q * (rinv * rinv * rinv) * nc.inv4pi;
 }
 
+// One lane per sorted evaluation point.  The vortices of a row of hash cells
+// (fixed y, z; x from i-K-1 to i+K+1 around the point's cell i) are one
+// contiguous sorted segment.  Lanes whose points share a cell run identical
+// loops and read the same vortices (broadcast loads); a tighter per-point
+// range (the point's box, 2K+2 cells per axis) measured slower because it
+// breaks that lockstep.
 template <int KERNEL, bool SEGMENT, typename TO>
 __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epos, const float4* __restrict__ eoth,
                                                   const uint32_t* __restrict__ eidx, uint32_t m,
@@ -89,36 +93,40 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
         }
     }
 
-    // ---- near field over hash rows
+    // ---- near field over the hash rows of the point's box
     float vx = 0.f, vy = 0.f, vz = 0.f;
-    float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (SEGMENT) bpos = eoth[t];
-    const int R = K + 1;
-    const uint32_t nsrc = off[G];
-    const int ilo = max(i - R, 0), ihi = min(i + R, gi - 1);
-    if (K >= 0 && ilo <= ihi) {
-        for (int kk = max(k - R, 0); kk <= min(k + R, gi - 1); kk++) {
-            for (int jj = max(j - R, 0); jj <= min(j + R, gi - 1); jj++) {
-                const uint32_t rowc = ((uint32_t)kk * g + (uint32_t)jj) * g;
-                const uint32_t p0 = off[rowc + ilo], p1 = min(off[rowc + ihi + 1], nsrc);
-                for (uint32_t p = p0; p < p1; p++) {
-                    const float4 s = spos[p];
-                    const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
-                    if (!(fabsf(dx) <= nc.rbox && fabsf(dy) <= nc.rbox && fabsf(dz) <= nc.rbox)) continue;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 < 1e-24f) continue;
-                    if (SEGMENT) {
-                        const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
-                        if (ex * ex + ey * ey + ez * ez < 1e-24f) continue;
+    if (K >= 0) {
+        float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (SEGMENT) bpos = eoth[t];
+        const int R = K + 1;
+        const int xlo = max(i - R, 0), xhi = min(i + R, gi - 1);
+        const int ylo = max(j - R, 0), yhi = min(j + R, gi - 1);
+        const int zlo = max(k - R, 0), zhi = min(k + R, gi - 1);
+        const uint32_t nsrc = off[G];
+        if (xlo <= xhi)
+            for (int kk = zlo; kk <= zhi; kk++) {
+                for (int jj = ylo; jj <= yhi; jj++) {
+                    const uint32_t rowc = ((uint32_t)kk * g + (uint32_t)jj) * g;
+                    const uint32_t p0 = off[rowc + xlo], p1 = min(off[rowc + xhi + 1], nsrc);
+                    for (uint32_t p = p0; p < p1; p++) {
+                        const float4 s = spos[p];
+                        const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
+                        if (!(fabsf(dx) <= nc.rbox && fabsf(dy) <= nc.rbox && fabsf(dz) <= nc.rbox)) continue;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 < 1e-24f) continue;
+                        if (SEGMENT) {
+                            const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
+                            if (ex * ex + ey * ey + ez * ez < 1e-24f) continue;
+                        }
+                        const float cf =
+                            KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc) : coeff_expcubic(r2, nc);
+                        const float4 o = somg[p];
+                        vx += (o.y * dz - o.z * dy) * cf;
+                        vy += (o.z * dx - o.x * dz) * cf;
+                        vz += (o.x * dy - o.y * dx) * cf;
                     }
-                    const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc) : coeff_expcubic(r2, nc);
-                    const float4 o = somg[p];
-                    vx += (o.y * dz - o.z * dy) * cf;
-                    vy += (o.z * dx - o.x * dz) * cf;
-                    vz += (o.x * dy - o.y * dx) * cf;
                 }
             }
-        }
     }
     const uint32_t dt = eidx[t];
     if (dt >= m) return;
@@ -132,13 +140,13 @@ template <int KERNEL, bool SEGMENT>
 static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st) {
     dim3 grid(div_up(a.m, 256)), blk(256);
     if (a.u_dtype == PPPM_F64)
-        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m, a.off,
-                           a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy, (double*)a.uz,
-                           a.u_stride);
+        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
+                           a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy,
+                           (double*)a.uz, a.u_stride);
     else
-        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, float>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m, a.off,
-                           a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (float*)a.ux, (float*)a.uy, (float*)a.uz,
-                           a.u_stride);
+        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, float>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
+                           a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (float*)a.ux, (float*)a.uy,
+                           (float*)a.uz, a.u_stride);
 }
 
 // largest float f with (double)f < x (x > 0; returns -1 otherwise)

@@ -283,3 +283,24 @@ extern "C" pppm_status pppm_mg_solve(pppm_mg* m, void* x, const void* b, int32_t
     return mg_solve_t(m->d, (double*)x, (const double*)b, schedule, nu1, nu2, omega, tol, max_cycles, cycles_out,
                       relres_out, st);
 }
+
+extern "C" pppm_status pppm_mg_smooth(pppm_mg* m, void* x, const void* b, int32_t iters, double omega,
+                                      int32_t blocked, void* stream) {
+    if (!m || !x || !b || iters < 0) FAIL(PPPM_ERR_INVALID, "bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (m->dtype == PPPM_F32) {
+        bool prev = m->f.blocked_smoother();
+        m->f.set_blocked_smoother(blocked != 0);
+        pppm_status s = m->f.smooth(0, (float*)x, (const float*)b, 1.0f, iters, (float)omega, 0.0f, st);
+        m->f.set_blocked_smoother(prev);
+        return s;
+    }
+    if (blocked) FAIL(PPPM_ERR_INVALID, "the blocked smoother is float only");
+    return m->d.smooth(0, (double*)x, (const double*)b, 1.0, iters, omega, 0.0, st);
+}
+
+extern "C" pppm_status pppm_mg_set_blocked(pppm_mg* m, int32_t blocked) {
+    if (!m) FAIL(PPPM_ERR_INVALID, "null mg");
+    m->f.set_blocked_smoother(blocked != 0);
+    return PPPM_OK;
+}

@@ -1,0 +1,387 @@
+// Request server: continuous batching of velocity evaluations on one GPU.
+//
+// The reference evaluates one scene at a time from one host thread
+// (VortexSimulationSystemImp.cpp:142-228, evaluateBiotSavart 1194-1244 in
+// fixed batches); here requests of any size and grid arrive concurrently.
+//
+// - A fixed number of slots (worker threads).  Each running request owns one
+//   pppm_ctx, and every context has its own stream, so the work of different
+//   requests overlaps on the device (for example one request's near field,
+//   VALU-bound, beside another's multigrid, HBM-bound).
+// - Contexts stay resident after a request and are reused by the next request
+//   with the same grid size and precision (hash tables, grids, multigrid
+//   hierarchy, captured multigrid graphs), re-targeted with
+//   pppm_update_params when the domain or schedule differs.
+// - Admission is FIFO against an HBM budget: the head request starts when a
+//   slot is free and its context fits (an idle context of its kind, or a new
+//   one after evicting least-recently-used idle contexts).  A request larger
+//   than the whole budget completes at once with PPPM_ERR_BUDGET.
+// - Input and output buffers belong to the caller; inputs must be complete on
+//   the device before submit, and both must stay valid until the request is
+//   done.
+// - Latency is measured per request from submit to completion.
+#include <algorithm>
+#include <cmath>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "pppm_internal.hpp"
+
+#define FAIL(code, msg)                                  \
+    do {                                                 \
+        pppm_set_last_error((msg), __FILE__, __LINE__);  \
+        return (code);                                   \
+    } while (0)
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+enum ReqState { kQueued = 0, kRunning = 1, kDone = 2 };
+
+struct Req {
+    pppm_request r{};
+    ReqState state = kQueued;
+    pppm_status result = PPPM_OK;
+    size_t need = 0;  // bytes of a fresh context for this request
+    double t_submit = 0, t_start = 0, t_end = 0;
+    pppm_stats stats{};
+};
+
+struct Slot {
+    pppm_ctx* ctx = nullptr;
+    uint32_t g = 0;
+    int32_t precision = 0;
+    size_t bytes = 0;  // device bytes held (or reserved while being created)
+    bool busy = false;
+    uint64_t last_use = 0;
+};
+
+bool same_params(const pppm_params& a, const pppm_params& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+}  // namespace
+
+struct pppm_server {
+    pppm_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Req> reqs;
+    std::vector<Slot*> pool;
+    size_t bytes = 0, peak_bytes = 0;
+    uint64_t next_ticket = 1, use_clock = 0;
+    int running = 0;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+    // statistics since the last reset
+    std::vector<double> lat_ms;
+    uint64_t submitted = 0, completed = 0, failed = 0, particles = 0;
+    uint64_t ctx_created = 0, ctx_reused = 0, ctx_evicted = 0;
+    double t_first = 0, t_last = 0;
+
+    // Evicts idle contexts other than keep, least recently used first, until
+    // `extra` more bytes fit in the budget.  False if they cannot (the rest
+    // is busy): the caller waits for a completion.
+    bool evict_for(size_t extra, const Slot* keep) {
+        while (bytes + extra > cfg.hbm_budget_bytes) {
+            Slot* lru = nullptr;
+            for (Slot* s : pool)
+                if (!s->busy && s != keep && (!lru || s->last_use < lru->last_use)) lru = s;
+            if (!lru) return false;
+            if (lru->ctx) pppm_destroy(lru->ctx);
+            bytes -= lru->bytes;
+            pool.erase(std::find(pool.begin(), pool.end(), lru));
+            delete lru;
+            ctx_evicted++;
+        }
+        return true;
+    }
+
+    // Chooses a context for the head request, under the lock.  Returns the
+    // slot (busy, possibly without a context yet) or null to wait.
+    Slot* admit(Req& q) {
+        const pppm_params& p = q.r.params;
+        Slot* best = nullptr;
+        for (Slot* s : pool)
+            if (!s->busy && s->ctx && s->g == p.g && s->precision == p.precision)
+                if (!best || s->last_use > best->last_use) best = s;
+        if (best) {
+            // the context grows to this request's size if it is larger
+            const size_t grow = q.need > best->bytes ? q.need - best->bytes : 0;
+            if (bytes + grow > cfg.hbm_budget_bytes && !evict_for(grow, best)) return nullptr;
+            best->busy = true;
+            best->last_use = ++use_clock;
+            best->bytes += grow;
+            bytes += grow;
+            peak_bytes = std::max(peak_bytes, bytes);
+            ctx_reused++;
+            return best;
+        }
+        // a new context
+        if (bytes + q.need > cfg.hbm_budget_bytes && !evict_for(q.need, nullptr)) return nullptr;
+        Slot* s = new Slot;
+        s->g = p.g;
+        s->precision = p.precision;
+        s->bytes = q.need;  // reservation until the context reports its size
+        s->busy = true;
+        s->last_use = ++use_clock;
+        pool.push_back(s);
+        bytes += q.need;
+        peak_bytes = std::max(peak_bytes, bytes);
+        return s;
+    }
+
+    // One request on one context; runs outside the lock.
+    pppm_status run(Slot* s, Req& q, bool* created) {
+        const pppm_request& r = q.r;
+        pppm_status st;
+        if (!s->ctx) {
+            if ((st = pppm_create(&r.params, &s->ctx)) != PPPM_OK) return st;
+            if (!cfg.use_graphs) (void)pppm_set_graphs(s->ctx, 0);
+            *created = true;
+        } else {
+            pppm_params cur;
+            (void)pppm_get_params(s->ctx, &cur);
+            if (!same_params(cur, r.params) && (st = pppm_update_params(s->ctx, &r.params)) != PPPM_OK) return st;
+        }
+        if ((st = pppm_set_vortices(s->ctx, r.n, r.pos4, r.wx, r.wy, r.wz, r.w_stride, r.w_dtype)) != PPPM_OK)
+            return st;
+        if ((st = pppm_compute_far_field(s->ctx)) != PPPM_OK) return st;
+        if (r.m == 0)
+            st = pppm_evaluate_self(s->ctx, r.ux, r.uy, r.uz, r.u_stride, r.u_dtype);
+        else
+            st = pppm_evaluate(s->ctx, r.m, r.eval_pos4, r.other_end4, r.ux, r.uy, r.uz, r.u_stride, r.u_dtype);
+        if (st != PPPM_OK) return st;
+        void* hs = nullptr;
+        (void)pppm_get_stream(s->ctx, &hs);
+        if (hipStreamSynchronize((hipStream_t)hs) != hipSuccess) FAIL(PPPM_ERR_HIP, "request stream failed");
+        (void)pppm_get_stats(s->ctx, &q.stats);
+        return PPPM_OK;
+    }
+
+    void worker() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            Slot* s = nullptr;
+            uint64_t t = 0;
+            cv_work.wait(lk, [&] {
+                if (stopping) return true;
+                if (queue.empty()) return false;
+                s = admit(reqs[queue.front()]);
+                return s != nullptr;
+            });
+            if (!s) return;  // stopping
+            t = queue.front();
+            queue.pop_front();
+            Req& q = reqs[t];
+            q.state = kRunning;
+            q.t_start = now_s();
+            running++;
+            lk.unlock();
+            bool created = false;
+            pppm_status st = run(s, q, &created);
+            size_t have = 0;
+            if (s->ctx) (void)pppm_get_device_bytes(s->ctx, &have);
+            lk.lock();
+            running--;
+            if (created) ctx_created++;
+            if (st != PPPM_OK && s->ctx) {
+                // a failed request may leave its context half-built: drop it
+                pppm_destroy(s->ctx);
+                s->ctx = nullptr;
+                have = 0;
+            }
+            bytes = bytes - s->bytes + have;
+            s->bytes = have;
+            peak_bytes = std::max(peak_bytes, bytes);
+            s->busy = false;
+            if (!s->ctx) {
+                pool.erase(std::find(pool.begin(), pool.end(), s));
+                delete s;
+            }
+            q.result = st;
+            q.t_end = now_s();
+            q.state = kDone;
+            finish(q);
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+
+    void finish(const Req& q) {
+        if (q.result == PPPM_OK) {
+            completed++;
+            particles += q.r.m ? q.r.m : q.r.n;
+            lat_ms.push_back(1e3 * (q.t_end - q.t_submit));
+        } else {
+            failed++;
+        }
+        t_last = std::max(t_last, q.t_end);
+    }
+};
+
+namespace {
+
+pppm_status check_request(const pppm_request* r) {
+    if (!r) FAIL(PPPM_ERR_INVALID, "null request");
+    if (r->n == 0 || !r->pos4 || !r->wx || !r->wy || !r->wz) FAIL(PPPM_ERR_INVALID, "request needs vortices");
+    if (!r->ux || !r->uy || !r->uz) FAIL(PPPM_ERR_INVALID, "request needs outputs");
+    if (r->m > 0 && !r->eval_pos4) FAIL(PPPM_ERR_INVALID, "request with m > 0 needs eval_pos4");
+    if (r->w_stride == 0 || r->u_stride == 0) FAIL(PPPM_ERR_INVALID, "strides must be >= 1");
+    return PPPM_OK;
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    // nearest rank
+    size_t k = (size_t)std::ceil(q * (double)v.size());
+    if (k < 1) k = 1;
+    if (k > v.size()) k = v.size();
+    return v[k - 1];
+}
+
+}  // namespace
+
+extern "C" pppm_status pppm_server_create(const pppm_server_config* cfg, pppm_server** out) {
+    if (!cfg || !out) FAIL(PPPM_ERR_INVALID, "null arg");
+    if (cfg->slots < 1 || cfg->slots > 16) FAIL(PPPM_ERR_INVALID, "slots must be in [1, 16]");
+    if (cfg->hbm_budget_bytes == 0) FAIL(PPPM_ERR_INVALID, "hbm_budget_bytes must be positive");
+    pppm_server* s = new pppm_server;
+    s->cfg = *cfg;
+    for (int i = 0; i < cfg->slots; i++) s->workers.emplace_back([s] { s->worker(); });
+    *out = s;
+    return PPPM_OK;
+}
+
+extern "C" void pppm_server_destroy(pppm_server* s) {
+    if (!s) return;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->queue.empty() && s->running == 0; });
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    for (Slot* sl : s->pool) {
+        if (sl->ctx) pppm_destroy(sl->ctx);
+        delete sl;
+    }
+    delete s;
+}
+
+extern "C" pppm_status pppm_request_estimate_bytes(const pppm_request* r, size_t* bytes) {
+    pppm_status st = check_request(r);
+    if (st != PPPM_OK) return st;
+    if (!bytes) FAIL(PPPM_ERR_INVALID, "null arg");
+    return pppm_estimate_device_bytes(&r->params, r->n, r->m, r->other_end4 != nullptr, bytes);
+}
+
+extern "C" pppm_status pppm_server_submit(pppm_server* s, const pppm_request* r, uint64_t* ticket) {
+    if (!s || !ticket) FAIL(PPPM_ERR_INVALID, "null arg");
+    pppm_status st = check_request(r);
+    if (st != PPPM_OK) return st;
+    size_t need = 0;
+    if ((st = pppm_estimate_device_bytes(&r->params, r->n, r->m, r->other_end4 != nullptr, &need)) != PPPM_OK)
+        return st;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const uint64_t t = s->next_ticket++;
+    Req& q = s->reqs[t];
+    q.r = *r;
+    q.need = need;
+    q.t_submit = now_s();
+    if (s->submitted == 0 || q.t_submit < s->t_first) s->t_first = q.t_submit;
+    s->submitted++;
+    *ticket = t;
+    if (need > s->cfg.hbm_budget_bytes) {
+        q.result = PPPM_ERR_BUDGET;
+        q.state = kDone;
+        q.t_end = q.t_submit;
+        s->finish(q);
+        s->cv_done.notify_all();
+        return PPPM_OK;
+    }
+    s->queue.push_back(t);
+    s->cv_work.notify_one();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_server_poll(pppm_server* s, uint64_t ticket, int32_t* done) {
+    if (!s || !done) FAIL(PPPM_ERR_INVALID, "null arg");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) FAIL(PPPM_ERR_INVALID, "unknown ticket");
+    *done = it->second.state == kDone;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_server_wait(pppm_server* s, uint64_t ticket, pppm_request_result* res) {
+    if (!s) FAIL(PPPM_ERR_INVALID, "null arg");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->reqs.find(ticket) == s->reqs.end()) FAIL(PPPM_ERR_INVALID, "unknown ticket");
+    // look the entry up again after every wake-up: inserts may rehash
+    s->cv_done.wait(lk, [&] { return s->reqs.find(ticket)->second.state == kDone; });
+    auto it = s->reqs.find(ticket);
+    const Req& q = it->second;
+    if (res) {
+        res->status = q.result;
+        res->latency_ms = 1e3 * (q.t_end - q.t_submit);
+        res->queue_ms = q.t_start > 0 ? 1e3 * (q.t_start - q.t_submit) : 0.0;
+        res->stats = q.stats;
+    }
+    s->reqs.erase(it);
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_server_drain(pppm_server* s) {
+    if (!s) FAIL(PPPM_ERR_INVALID, "null arg");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->queue.empty() && s->running == 0; });
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_server_get_stats(pppm_server* s, pppm_server_stats* out) {
+    if (!s || !out) FAIL(PPPM_ERR_INVALID, "null arg");
+    std::lock_guard<std::mutex> lk(s->mu);
+    memset(out, 0, sizeof(*out));
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->failed = s->failed;
+    out->particles = s->particles;
+    out->queued = (uint32_t)s->queue.size();
+    out->running = (uint32_t)s->running;
+    out->contexts = (uint32_t)s->pool.size();
+    out->ctx_created = s->ctx_created;
+    out->ctx_reused = s->ctx_reused;
+    out->ctx_evicted = s->ctx_evicted;
+    out->device_bytes = s->bytes;
+    out->peak_device_bytes = s->peak_bytes;
+    out->p50_ms = percentile(s->lat_ms, 0.50);
+    out->p99_ms = percentile(s->lat_ms, 0.99);
+    double sum = 0;
+    for (double v : s->lat_ms) sum += v;
+    out->mean_ms = s->lat_ms.empty() ? 0.0 : sum / (double)s->lat_ms.size();
+    out->wall_s = s->t_last > s->t_first ? s->t_last - s->t_first : 0.0;
+    out->particles_per_s = out->wall_s > 0 ? (double)s->particles / out->wall_s : 0.0;
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_server_reset_stats(pppm_server* s) {
+    if (!s) FAIL(PPPM_ERR_INVALID, "null arg");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat_ms.clear();
+    s->submitted = s->completed = s->failed = s->particles = 0;
+    s->ctx_created = s->ctx_reused = s->ctx_evicted = 0;
+    s->t_first = s->t_last = 0;
+    s->peak_bytes = s->bytes;
+    return PPPM_OK;
+}

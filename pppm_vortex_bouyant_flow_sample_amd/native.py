@@ -205,6 +205,18 @@ class Context:
         check(lib().pppm_get_stats(self.h, C.byref(s)), "get_stats")
         return s
 
+    def update_params(self, params):
+        check(lib().pppm_update_params(self.h, C.byref(params)), "update_params")
+        self.params = params
+
+    def set_graphs(self, on):
+        check(lib().pppm_set_graphs(self.h, C.c_int32(1 if on else 0)), "set_graphs")
+
+    def device_bytes(self):
+        b = C.c_size_t()
+        check(lib().pppm_get_device_bytes(self.h, C.byref(b)), "get_device_bytes")
+        return b.value
+
     def sorted_vortices(self):
         pos, idx, st, en = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
         check(lib().pppm_get_sorted_vortices(self.h, C.byref(pos), C.byref(idx), C.byref(st), C.byref(en)),
@@ -253,6 +265,106 @@ def solve_self(ctx, scene, u_dtype=F64):
     synchronize()
     dt = np.float64 if u_dtype == F64 else np.float32
     return np.stack([b.to_array(dt, scene.n).astype(np.float64) for b in out])
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("slots", C.c_int32), ("use_graphs", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("params", Params), ("n", C.c_uint32), ("pos4", C.c_void_p),
+        ("wx", C.c_void_p), ("wy", C.c_void_p), ("wz", C.c_void_p), ("w_stride", C.c_uint32), ("w_dtype", C.c_int32),
+        ("m", C.c_uint32), ("eval_pos4", C.c_void_p), ("other_end4", C.c_void_p),
+        ("ux", C.c_void_p), ("uy", C.c_void_p), ("uz", C.c_void_p), ("u_stride", C.c_uint32), ("u_dtype", C.c_int32),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("pad", C.c_int32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+                ("stats", Stats)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("particles", C.c_uint64),
+        ("queued", C.c_uint32), ("running", C.c_uint32), ("contexts", C.c_uint32), ("pad", C.c_uint32),
+        ("ctx_created", C.c_uint64), ("ctx_reused", C.c_uint64), ("ctx_evicted", C.c_uint64),
+        ("device_bytes", C.c_uint64), ("peak_device_bytes", C.c_uint64),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("wall_s", C.c_double),
+        ("particles_per_s", C.c_double),
+    ]
+
+
+def make_request(params, n, pos_ptr, w_ptrs, u_ptrs, w_dtype=F64, u_dtype=F64, m=0, eval_ptr=None, other_ptr=None,
+                 w_stride=1, u_stride=1):
+    r = Request()
+    r.params = params
+    r.n = n
+    r.pos4 = pos_ptr
+    r.wx, r.wy, r.wz = w_ptrs
+    r.w_stride, r.w_dtype = w_stride, w_dtype
+    r.m = m
+    r.eval_pos4 = eval_ptr
+    r.other_end4 = other_ptr
+    r.ux, r.uy, r.uz = u_ptrs
+    r.u_stride, r.u_dtype = u_stride, u_dtype
+    return r
+
+
+def estimate_bytes(params, n, m=0, other_end=False):
+    b = C.c_size_t()
+    check(lib().pppm_estimate_device_bytes(C.byref(params), C.c_uint32(n), C.c_uint32(m), C.c_int32(int(other_end)),
+                                           C.byref(b)), "estimate_device_bytes")
+    return b.value
+
+
+class Server:
+    """pppm_server: continuous batching of evaluation requests."""
+
+    def __init__(self, hbm_budget_bytes, slots=2, use_graphs=True):
+        cfg = ServerConfig(hbm_budget_bytes, slots, 1 if use_graphs else 0)
+        self.h = C.c_void_p()
+        L = lib()
+        L.pppm_server_destroy.restype = None
+        check(L.pppm_server_create(C.byref(cfg), C.byref(self.h)), "server_create")
+
+    def submit(self, req):
+        t = C.c_uint64()
+        check(lib().pppm_server_submit(self.h, C.byref(req), C.byref(t)), "server_submit")
+        return t.value
+
+    def poll(self, ticket):
+        d = C.c_int32()
+        check(lib().pppm_server_poll(self.h, C.c_uint64(ticket), C.byref(d)), "server_poll")
+        return bool(d.value)
+
+    def wait(self, ticket):
+        r = RequestResult()
+        check(lib().pppm_server_wait(self.h, C.c_uint64(ticket), C.byref(r)), "server_wait")
+        return r
+
+    def drain(self):
+        check(lib().pppm_server_drain(self.h), "server_drain")
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().pppm_server_get_stats(self.h, C.byref(s)), "server_get_stats")
+        return s
+
+    def reset_stats(self):
+        check(lib().pppm_server_reset_stats(self.h), "server_reset_stats")
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().pppm_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Timer:

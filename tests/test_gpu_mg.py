@@ -58,3 +58,38 @@ def test_mg_fp32_reaches_working_precision(native, oracle):
     assert rel <= 1e-5, (cyc, rel)
     x = bx.to_array(np.float32, xs.size)
     assert rel_l2(x, xs) < 1e-4
+
+
+@pytest.mark.parametrize("dims,iters", [((64, 64, 64), 2), ((48, 40, 72), 1), ((256, 32, 20), 3), ((9, 7, 5), 2)])
+def test_blocked_smoother_equals_halfsweep_kernel(native, dims, iters):
+    """The temporally blocked LDS smoother performs exactly the red-black
+    half-sweeps of the one-launch-per-colour kernel (FP32, rounding only)."""
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(sum(dims))
+    x0 = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    mg = native.MG(dims, native.F32)
+    out = []
+    for blocked in (0, 1):
+        bx, bb = native.DeviceBuffer.from_array(x0), native.DeviceBuffer.from_array(b)
+        native.check(native.lib().pppm_mg_smooth(mg.h, bx.ptr, bb.ptr, iters, native.C.c_double(1.0), blocked, None))
+        native.synchronize()
+        out.append(bx.to_array(np.float32, n))
+    assert np.max(np.abs(out[0] - out[1])) <= 1e-5 * np.max(np.abs(out[0]))
+
+
+@pytest.mark.parametrize("dims,nu", [((128, 128, 128), (2, 2)), ((96, 64, 40), (1, 2)), ((32, 32, 32), (2, 1))])
+def test_fused_vcycle_equals_unfused(native, oracle, dims, nu):
+    """One fused V-cycle (blocked smoothing with residual/restriction and
+    prolongation folded in) equals the launch-per-step V-cycle to rounding."""
+    xs, b = _manufactured(oracle, dims, 9)
+    b = b.astype(np.float32)
+    res = []
+    for blocked in (0, 1):
+        mg = native.MG(dims, native.F32)
+        native.check(native.lib().pppm_mg_set_blocked(mg.h, blocked))
+        bx, bb = native.DeviceBuffer(4 * b.size), native.DeviceBuffer.from_array(b)
+        bx.fill_zero()
+        mg.solve(bx.ptr, bb.ptr, native.MG_VCYCLE, nu[0], nu[1], 1.0, 0.0, 2)
+        res.append(bx.to_array(np.float32, b.size))
+    assert rel_l2(res[1], res[0]) < 1e-5
