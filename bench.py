@@ -6,16 +6,24 @@ evaluation at every vortex: hash + sort + reorder, P2M, three Poisson solves,
 curl + far-field buffer, M2P + near field + unsort.  Inputs (float4 positions,
 float32 strengths) are resident in HBM before the timed region.
 
-Two paths are timed on the same inputs:
-  mixed : FP32 multigrid inside FP64 defect correction (tolerance 1e-9,
-          velocity error <= 1e-5 vs the FP64 oracle in the parity tests)
-  fp32  : plain FP32 path (V-cycles to 1e-6 or the FP32 rounding floor)
-`value` is the mixed path's particle-velocities/s; the plain FP32 figure and
-per-step p50/p99 latencies are reported beside it.
+Three paths are timed on the same inputs:
+  served : the mixed-precision path behind the request server (continuous
+           batching): `--slots` requests of this workload are kept in flight,
+           each on its own resident context and stream, so one request's
+           near field (VALU-bound) overlaps another's multigrid (HBM-bound).
+           One step = one request; latency is per request, submit to done.
+  mixed  : the same mixed-precision path, one evaluation after another
+           (FP32 multigrid inside FP64 defect correction, tolerance 1e-9,
+           velocity error <= 1e-5 vs the FP64 oracle in the parity tests)
+  fp32   : the plain FP32 path, one evaluation after another (V-cycles to 1e-6
+           or the FP32 rounding floor)
+`value` is the served path's particle-velocities/s; the plain FP32 and
+sequential mixed figures and their p50/p99 latencies are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--slots S] [--no-cpu-baseline]
 """
 import argparse
+import collections
 import json
 import os
 import sys
@@ -38,7 +46,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plume-8M-256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="mixed,fp32")
+    ap.add_argument("--paths", default="served,mixed,fp32")
+    ap.add_argument("--slots", type=int, default=3)
     return ap.parse_args()
 
 
@@ -87,12 +96,61 @@ def run_path(N, scene, g, prec, steps, warmup):
     wall = time.perf_counter() - t0
     stats = ctx.stats()
     pairs = ctx.h and _count_pairs(N, ctx)
+    u_last = np.stack([b.to_array(np.float32, n) for b in out])
     ctx.close()
     return {
+        "u_last": u_last,
         "wall_s": wall, "lat_ms": [1e3 * x for x in lat], "hash_ms": th, "far_ms": tf, "near_ms": tn,
         "mg_cycles": stats.mg_cycles, "mg_outer": stats.mg_outer, "mg_relres": list(stats.mg_relres),
         "mg_converged": bool(stats.mg_converged), "pairs": pairs,
     }
+
+
+def run_served(N, scene, g, steps, warmup, slots):
+    """Closed loop through the request server: `slots` requests outstanding,
+    a new one submitted as each completes."""
+    p = N.config_params(g, precision=N.PREC_MIXED, mg_tol=1e-9, mg_max_cycles=40)
+    n = scene.n
+    outs = [[N.DeviceBuffer(4 * n) for _ in range(3)] for _ in range(slots)]
+    budget = (slots + 1) * N.estimate_bytes(p, n)
+    srv = N.Server(budget, slots=slots)
+
+    def request(o):
+        return N.make_request(p, n, scene.pos.ptr, scene.w_ptrs(), [b.ptr for b in outs[o]],
+                              w_dtype=scene.w_dtype, u_dtype=N.F32)
+
+    last = [None]
+
+    def run(count):
+        pending, free, submitted = collections.deque(), list(range(slots)), 0
+        while submitted < count or pending:
+            while free and submitted < count:
+                o = free.pop()
+                pending.append((srv.submit(request(o)), o))
+                submitted += 1
+            t, o = pending.popleft()
+            r = srv.wait(t)
+            if r.status != 0:
+                raise RuntimeError(f"served request failed: {N.STATUS.get(r.status)}")
+            last[0] = (o, r)
+            free.append(o)
+
+    run(warmup)
+    srv.reset_stats()
+    N.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    N.synchronize()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    o, r = last[0]
+    u_last = np.stack([b.to_array(np.float32, n) for b in outs[o]])
+    out = {"wall_s": wall, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "mean_ms": st.mean_ms,
+           "contexts": st.contexts, "ctx_created": st.ctx_created, "ctx_reused": st.ctx_reused,
+           "peak_device_bytes": st.peak_device_bytes, "budget_bytes": budget,
+           "mg_outer": r.stats.mg_outer, "mg_relres": max(r.stats.mg_relres), "u_last": u_last}
+    srv.close()
+    return out
 
 
 def _count_pairs(N, ctx):
@@ -101,6 +159,20 @@ def _count_pairs(N, ctx):
     v = C.c_uint64()
     N.check(N.lib().pppm_count_near_pairs(ctx.h, C.byref(v)), "count_near_pairs")
     return int(v.value)
+
+
+def near_traffic():
+    """HBM bytes per launch of the near-field kernel on this workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled
+    by the same counter's reading on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "near_field_plume8M_256_counters.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            vals["t"] = float(line.split("=")[1].split()[0])
+    return vals.get("t")
 
 
 def cpu_baseline():
@@ -139,6 +211,12 @@ def main():
     scene = N.Scene(pos, w.astype(np.float32), N.F32)
     res = {}
     for name in a.paths.split(","):
+        if name == "served":
+            log(f"[bench] path served: {a.slots} slots, warmup {a.warmup}, steps {a.steps}")
+            res[name] = r = run_served(N, scene, g, a.steps, a.warmup, a.slots)
+            log(f"[bench]   {1e3 * r['wall_s'] / a.steps:.2f} ms/request amortised; latency p50 {r['p50_ms']:.1f} "
+                f"p99 {r['p99_ms']:.1f} ms; outer {r['mg_outer']} relres {r['mg_relres']:.2e}")
+            continue
         prec = N.PREC_MIXED if name == "mixed" else N.PREC_FP32
         log(f"[bench] path {name}: warmup {a.warmup}, steps {a.steps}")
         res[name] = run_path(N, scene, g, prec, a.steps, a.warmup)
@@ -146,38 +224,51 @@ def main():
         log(f"[bench]   {1e3 * r['wall_s'] / a.steps:.2f} ms/step; hash {np.median(r['hash_ms']):.2f} "
             f"far {np.median(r['far_ms']):.2f} near {np.median(r['near_ms']):.2f} ms; "
             f"mg cycles {r['mg_cycles']} outer {r['mg_outer']} relres {max(r['mg_relres']):.2e}")
-    head = "mixed" if "mixed" in res else list(res)[0]
+    head = "served" if "served" in res else ("mixed" if "mixed" in res else list(res)[0])
     r = res[head]
     ms = 1e3 * r["wall_s"] / a.steps
     value = n / (r["wall_s"] / a.steps)
-    t_near = float(np.mean(r["near_ms"])) * 1e-3
-    achieved = FLOP_PER_PAIR * r["pairs"] / t_near / 1e12
+    seq = res.get("mixed") or res.get("fp32")
+    t_near = float(np.mean(seq["near_ms"])) * 1e-3
+    achieved = FLOP_PER_PAIR * seq["pairs"] / t_near / 1e12
+    lat = ({"p50": r["p50_ms"], "p99": r["p99_ms"]} if head == "served" else
+           {"p50": float(np.percentile(r["lat_ms"], 50)), "p99": float(np.percentile(r["lat_ms"], 99))})
     out = {
         "metric": "PPPM particle-velocities/s at 8M/256^3 vs plain FP32 GPU; p50/p99 latency",
         "value": value, "unit": "particle-velocities/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f32", "data": "synthetic",
+        "dtype": "f32+f64" if head in ("served", "mixed") else "f32", "data": "synthetic",
         "config": {"workload": f"{a.workload} (plume: x,y~N(0.5,0.08), z~Exp(3), w~N(0,1)e^-3z)",
                    "vortices": n, "grid": g, "K": 2, "core": "expcubic 1.5h", "multigrid": "V(2,2) RB-GS",
-                   "path": head, "parallelism": "1 GPU"},
-        "latency_ms": {"p50": float(np.percentile(r["lat_ms"], 50)), "p99": float(np.percentile(r["lat_ms"], 99))},
-        "stages_ms": {"hash": float(np.median(r["hash_ms"])), "p2m_mg_far": float(np.median(r["far_ms"])),
-                      "m2p_near": float(np.median(r["near_ms"]))},
-        "mg": {"cycles": r["mg_cycles"], "outer": r["mg_outer"], "relres": max(r["mg_relres"])},
+                   "path": head + (f" ({a.slots} slots, mixed precision)" if head == "served" else ""),
+                   "parallelism": "1 GPU"},
+        "latency_ms": lat,
         "roofline": {"bound": "valu", "kernel": "k_m2p_near", "achieved": achieved,
                      "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": None,
-                     "pairs": r["pairs"], "flop_per_pair": FLOP_PER_PAIR},
+                     "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "traffic": near_traffic(),
+                     "pairs": seq["pairs"], "flop_per_pair": FLOP_PER_PAIR,
+                     "avg_launch_ms": 1e3 * t_near},
     }
+    if head == "served":
+        out["served"] = {k: r[k] for k in ("mean_ms", "contexts", "ctx_created", "ctx_reused",
+                                           "peak_device_bytes", "budget_bytes", "mg_outer", "mg_relres")}
+        if "mixed" in res:
+            out["served"]["equals_sequential_mixed"] = bool(np.array_equal(r["u_last"], res["mixed"]["u_last"]))
     for other in res:
-        if other != head:
-            ro = res[other]
-            out[f"{other}_path"] = {
-                "value": n / (ro["wall_s"] / a.steps), "ms_per_step": 1e3 * ro["wall_s"] / a.steps,
-                "latency_ms": {"p50": float(np.percentile(ro["lat_ms"], 50)),
-                               "p99": float(np.percentile(ro["lat_ms"], 99))},
-                "mg": {"cycles": ro["mg_cycles"], "relres": max(ro["mg_relres"]), "converged": ro["mg_converged"]},
-            }
+        if other == head:
+            continue
+        ro = res[other]
+        out[f"{other}_path"] = {
+            "value": n / (ro["wall_s"] / a.steps), "ms_per_step": 1e3 * ro["wall_s"] / a.steps,
+            "latency_ms": {"p50": float(np.percentile(ro["lat_ms"], 50)),
+                           "p99": float(np.percentile(ro["lat_ms"], 99))},
+            "stages_ms": {"hash": float(np.median(ro["hash_ms"])), "p2m_mg_far": float(np.median(ro["far_ms"])),
+                          "m2p_near": float(np.median(ro["near_ms"]))},
+            "mg": {"cycles": ro["mg_cycles"], "outer": ro["mg_outer"], "relres": max(ro["mg_relres"]),
+                   "converged": ro["mg_converged"]},
+        }
+    if "fp32" in res:
+        out["vs_plain_fp32"] = value / out["fp32_path"]["value"] if head != "fp32" else 1.0
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

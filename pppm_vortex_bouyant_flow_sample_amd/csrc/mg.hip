@@ -138,9 +138,14 @@ __global__ void __launch_bounds__(256) k_rb2(TbArgs a) {
     __shared__ float bs[NB * PL];
     __shared__ float cw[PROLONG ? 3 : 1][CLY][CLX];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int gx0 = blockIdx.x * kTbTX - H, gy0 = blockIdx.y * kTbTY - H;
+    // tiles in x-fastest order, a contiguous range of them per XCD
+    const uint32_t tile = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                    gridDim.x * gridDim.y * gridDim.z);
+    const int bx = (int)(tile % gridDim.x), by = (int)((tile / gridDim.x) % gridDim.y);
+    const int bz = (int)(tile / (gridDim.x * gridDim.y));
+    const int gx0 = bx * kTbTX - H, gy0 = by * kTbTY - H;
     const int nzc = (nz + a.zc - 1) / a.zc;
-    const int comp = blockIdx.z / nzc, zi = blockIdx.z - comp * nzc;
+    const int comp = bz / nzc, zi = bz - comp * nzc;
     const int z0 = zi * a.zc, z1 = min(z0 + a.zc, nz);
     const float* xsrc = a.xin + (size_t)comp * a.G;
     float* xc = a.x + (size_t)comp * a.G;
@@ -330,7 +335,7 @@ __global__ void __launch_bounds__(256) k_rb2(TbArgs a) {
                 if ((pr & 1) == 0) {
                     rsum = part;
                 } else {
-                    const int gI = blockIdx.x * (kTbTX / 2) + I, gJ = blockIdx.y * (kTbTY / 2) + J, gK = pr / 2;
+                    const int gI = bx * (kTbTX / 2) + I, gJ = by * (kTbTY / 2) + J, gK = pr / 2;
                     if (gI < a.cx && gJ < a.cy && gK < a.cz)
                         a.bn[(size_t)comp * a.Gc + ((size_t)gK * a.cy + gJ) * a.cx + gI] = (rsum + part) * 0.5f;
                 }
@@ -537,60 +542,93 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* __restric
     }
 }
 
-// FP64 defect-correction step (see mg.hpp).
+// FP64 defect-correction step (see mg.hpp).  A workgroup owns a 64 x 4 tile
+// of x columns and a run of z planes; each thread marches its column in z
+// with x at k-1, k, k+1 in registers (x = x_in + e, formed once per cell) and
+// takes the in-plane neighbours from an LDS plane with a one-cell halo (the
+// first 140 threads also form the halo cells).  Norm partial sums are per
+// workgroup, reduced afterwards in a fixed order.
+constexpr int kDcTX = 64, kDcTY = 4;
+
 __global__ void __launch_bounds__(256) k_dc_step(const double* __restrict__ xin, const float* __restrict__ e,
                                                  const double* __restrict__ b64, double* __restrict__ xout,
-                                                 float* __restrict__ r32, int nx, int ny, int nz, size_t G,
+                                                 float* __restrict__ r32, int nx, int ny, int nz, size_t G, int zc,
                                                  double* __restrict__ part) {
+    __shared__ double pl[kDcTY + 2][kDcTX + 2];
     __shared__ double lds[4];
-    const int comp = blockIdx.y;
-    const double* xi = xin ? xin + (size_t)comp * G : nullptr;
-    const float* ec = e ? e + (size_t)comp * G : nullptr;
+    const int tilesx = (nx + kDcTX - 1) / kDcTX, tilesy = (ny + kDcTY - 1) / kDcTY;
+    const int nzc = (nz + zc - 1) / zc;
+    const int blk = blockIdx.x;  // ((comp * nzc + zchunk) * tilesy + ty) * tilesx + tx
+    const int tx = blk % tilesx, ty = (blk / tilesx) % tilesy, zq = (blk / (tilesx * tilesy)) % nzc;
+    const int comp = blk / (tilesx * tilesy * nzc);
+    const int lx = threadIdx.x % kDcTX, ly = threadIdx.x / kDcTX;
+    const int i = tx * kDcTX + lx, j = ty * kDcTY + ly;
+    const int z0 = zq * zc, z1 = min(z0 + zc, nz);
+    const size_t cb = (size_t)comp * G;
+    const bool own = i < nx && j < ny;
+    auto X = [&](int ii, int jj, int kk) -> double {
+        if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kk < 0 || kk >= nz) return 0.0;
+        const size_t q = cb + ((size_t)kk * ny + jj) * nx + ii;
+        return (xin ? xin[q] : 0.0) + (e ? (double)e[q] : 0.0);
+    };
+    // halo cell of this thread (threads 0..139): the ring around the tile
+    int hx = -2, hy = -2;
+    {
+        const int h = threadIdx.x;
+        if (h < kDcTX + 2) { hx = h - 1; hy = -1; }
+        else if (h < 2 * (kDcTX + 2)) { hx = h - (kDcTX + 2) - 1; hy = kDcTY; }
+        else if (h < 2 * (kDcTX + 2) + kDcTY) { hx = -1; hy = h - 2 * (kDcTX + 2); }
+        else if (h < 2 * (kDcTX + 2) + 2 * kDcTY) { hx = kDcTX; hy = h - 2 * (kDcTX + 2) - kDcTY; }
+    }
+    const bool halo = hx > -2;
+    const int hi = tx * kDcTX + hx, hj = ty * kDcTY + hy;
+    double xm = own ? X(i, j, z0 - 1) : 0.0, xc = own ? X(i, j, z0) : 0.0;
     double rr = 0, bb = 0;
-    auto X = [&](size_t q) { return (xi ? xi[q] : 0.0) + (ec ? (double)ec[q] : 0.0); };
-    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-    // block b takes a contiguous run of x rows; lanes run along x
-    const int rows = ny * nz, per = (rows + gridDim.x - 1) / gridDim.x;
-    const int r0 = blockIdx.x * per, r1 = min(r0 + per, rows);
-    for (int row = r0; row < r1; row++) {
-        const int j = row % ny, k = row / ny;
-        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-            const size_t t = (size_t)row * nx + i;
-            double xc = X(t);
-            double lv = i > 0 ? X(t - 1) : 0.0;
-            double rv = i < nx - 1 ? X(t + 1) : 0.0;
-            double uv = j < ny - 1 ? X(t + sy) : 0.0;
-            double dv = j > 0 ? X(t - sy) : 0.0;
-            double fv = k > 0 ? X(t - sz) : 0.0;
-            double bv = k < nz - 1 ? X(t + sz) : 0.0;
-            double s = lv + rv;
-            s += uv + dv;
-            s += fv + bv;
+    for (int k = z0; k < z1; k++) {
+        const double xp = own ? X(i, j, k + 1) : 0.0;
+        pl[ly + 1][lx + 1] = xc;
+        if (halo) pl[hy + 1][hx + 1] = X(hi, hj, k);
+        __syncthreads();
+        if (own) {
+            double s = pl[ly + 1][lx] + pl[ly + 1][lx + 2];
+            s += pl[ly][lx + 1] + pl[ly + 2][lx + 1];
+            s += xm + xp;
             s -= 6.0 * xc;
-            double bval = b64[(size_t)comp * G + t];
-            double r = bval - s;
-            xout[(size_t)comp * G + t] = xc;
-            r32[(size_t)comp * G + t] = (float)r;
+            const size_t q = cb + ((size_t)k * ny + j) * nx + i;
+            const double bval = b64[q];
+            const double r = bval - s;
+            xout[q] = xc;
+            r32[q] = (float)r;
             rr += r * r;
             bb += bval * bval;
         }
+        __syncthreads();
+        xm = xc;
+        xc = xp;
     }
-    double s1 = block_sum_256(rr, lds);
-    double s2 = block_sum_256(bb, lds);
+    const double s1 = block_sum_256(rr, lds);
+    const double s2 = block_sum_256(bb, lds);
     if (threadIdx.x == 0) {
-        part[((size_t)comp * gridDim.x + blockIdx.x) * 2] = s1;
-        part[((size_t)comp * gridDim.x + blockIdx.x) * 2 + 1] = s2;
+        const int per = tilesx * tilesy * nzc, b = blk - comp * per;
+        part[((size_t)comp * per + b) * 2] = s1;
+        part[((size_t)comp * per + b) * 2 + 1] = s2;
     }
 }
 
 pppm_status launch_dc_step(const double* x_in, const float* e, const double* b64, double* x_out, float* r32, int nx,
                            int ny, int nz, int ncomp, double* partials, double* out, hipStream_t st) {
     const size_t G = (size_t)nx * ny * nz;
-    const int nblk = MGHier<float>::kNormBlocks;
-    hipLaunchKernelGGL(k_dc_step, dim3(nblk, ncomp), dim3(256), 0, st, x_in, e, b64, x_out, r32, nx, ny, nz, G,
+    const int tiles = ((nx + kDcTX - 1) / kDcTX) * ((ny + kDcTY - 1) / kDcTY);
+    // as many z chunks as fit the partial-sum slots of one component
+    const int nzc = std::max(1, std::min(nz, MGHier<float>::kNormBlocks / tiles));
+    if (tiles > MGHier<float>::kNormBlocks) return PPPM_ERR_INVALID;
+    const int zc = (nz + nzc - 1) / nzc;
+    const int nzc_eff = (nz + zc - 1) / zc;
+    const int per = tiles * nzc_eff;
+    hipLaunchKernelGGL(k_dc_step, dim3(per * ncomp), dim3(256), 0, st, x_in, e, b64, x_out, r32, nx, ny, nz, G, zc,
                        partials);
     PPPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(256), 0, st, partials, nblk, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp), dim3(256), 0, st, partials, per, out);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }
@@ -799,7 +837,16 @@ pppm_status MGHier<T>::copy(T* d, const T* s, size_t n, hipStream_t st) {
     return PPPM_OK;
 }
 
-static int tb_zchunk(int nz) { return nz >= 128 ? 32 : (nz >= 32 ? 16 : nz + (nz & 1)); }
+// z planes per workgroup of k_rb2: even (restriction pairs), at least 2,
+// and small enough that a pass has about 1024 workgroups; the coarse levels
+// are latency-bound, so parallelism beats the extra halo planes there.
+static int tb_zchunk(int nx, int ny, int nz, int ncomp) {
+    const int tiles = (int)(div_up(nx, kTbTX) * div_up(ny, kTbTY)) * ncomp;
+    const int want = std::max(1, (1024 + tiles - 1) / tiles);
+    int zc = (nz + want - 1) / want;
+    zc += zc & 1;
+    return std::max(2, std::min(zc, nz + (nz & 1)));
+}
 
 // One launch of k_rb2 over all components.
 template <bool R, bool P, bool Z>
@@ -833,7 +880,7 @@ pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float b
     a.G = G;
     a.omega = omega;
     a.beta = beta;
-    a.zc = tb_zchunk(nz);
+    a.zc = tb_zchunk(nx, ny, nz, ncomp);
     return launch_rb2<false, false, false>(a, ncomp, st);
 }
 
@@ -947,7 +994,7 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
         a.G = m.cells(l);
         a.omega = omega;
         a.beta = (float)mg_beta(l);
-        a.zc = tb_zchunk(d[2]);
+        a.zc = tb_zchunk(d[0], d[1], d[2], nc);
         a.cx = c[0];
         a.cy = c[1];
         a.cz = c[2];
@@ -1035,10 +1082,10 @@ template <typename T>
 pppm_status MGHier<T>::norms_launch(const T* x0, const T* b0, T bscale0, hipStream_t st) {
     const int nx = d_[0][0], ny = d_[0][1], nz = d_[0][2];
     double* out = part_ + (size_t)kNormBlocks * 3 * 2;
-    hipLaunchKernelGGL(k_resnorm<T>, dim3(kNormBlocks, ncomp_), dim3(256), 0, st, x0, b0, bscale0, nx, ny, nz,
+    hipLaunchKernelGGL(k_resnorm<T>, dim3(kResBlocks, ncomp_), dim3(256), 0, st, x0, b0, bscale0, nx, ny, nz,
                        cells(0), part_);
     PPPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp_), dim3(256), 0, st, part_, kNormBlocks, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncomp_), dim3(256), 0, st, part_, kResBlocks, out);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }

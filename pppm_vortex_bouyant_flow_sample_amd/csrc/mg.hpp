@@ -60,7 +60,10 @@ class MGHier {
     T* temp(int l) { return t_[l]; }
     double* partials() { return part_; }
     double* host_norms() { return hnorm_; }
-    static constexpr int kNormBlocks = 512;
+    // partial-sum slots per component (the defect-correction step uses one
+    // per workgroup: up to 4096 at 1024^3); the residual norm uses kResBlocks
+    static constexpr int kNormBlocks = 4096;
+    static constexpr int kResBlocks = 512;
     // use the temporally blocked smoother (float only)
     void set_blocked_smoother(bool on) { tb_ = on; }
     bool blocked_smoother() const { return tb_; }

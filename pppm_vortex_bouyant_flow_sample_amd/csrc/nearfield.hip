@@ -44,12 +44,30 @@ __device__ inline float coeff_gaussian(float r2, const NearConst& nc) {
     return q * (rinv * rinv * rinv) * nc.inv4pi;
 }
 
-// One lane per sorted evaluation point.  The vortices of a row of hash cells
-// (fixed y, z; x from i-K-1 to i+K+1 around the point's cell i) are one
-// contiguous sorted segment.  Lanes whose points share a cell run identical
-// loops and read the same vortices (broadcast loads); a tighter per-point
-// range (the point's box, 2K+2 cells per axis) measured slower because it
-// breaks that lockstep.
+// Cell range [lo, hi] along one axis holding every vortex whose coordinate is
+// within rb of v.  A vortex's hash cell equals floor of its coordinate over h
+// up to FP32 rounding of the hash (far below 1e-3 cells), so the margin keeps
+// the range a superset of the box; the box test decides.
+__device__ inline int near_lo_cell(float v, double rb, double invh) {
+    return max(d2i_sat(floor(((double)v - rb) * invh - 1e-3)), 0);
+}
+__device__ inline int near_hi_cell(float v, double rb, double invh, int gi) {
+    return min(d2i_sat(floor(((double)v + rb) * invh + 1e-3)), gi - 1);
+}
+
+// Cluster form: one wave takes kNearTI consecutive sorted evaluation points
+// (usually one hash cell) and splits the vortices among kNearSJ lanes per
+// point (lane = point + kNearTI * slice).  It scans the hash rows covering the
+// union of the points' boxes; each row segment (consecutive x cells are
+// consecutive sorted vortices) is staged 64 vortices at a time into LDS with
+// coalesced loads, and each lane tests its slice of the chunk against its
+// point, evaluating the kernel where the box test passes.  The M2P corners of
+// a point are shared among its lanes; partial sums are reduced over the
+// slices at the end (fixed order, deterministic).  All loop bounds are
+// wave-uniform.
+constexpr int kNearTI = 4;
+constexpr int kNearSJ = 64 / kNearTI;
+
 template <int KERNEL, bool SEGMENT, typename TO>
 __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epos, const float4* __restrict__ eoth,
                                                   const uint32_t* __restrict__ eidx, uint32_t m,
@@ -57,88 +75,127 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                                                   const float4* __restrict__ somg, const float* __restrict__ far,
                                                   uint32_t g, int K, double invh, NearConst nc, TO* __restrict__ ux,
                                                   TO* __restrict__ uy, TO* __restrict__ uz, uint32_t u_stride) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
+    __shared__ float4 cpos[4][64], comg[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ti = lane % kNearTI, sj = lane / kNearTI;
+    const uint32_t t0 = (blockIdx.x * 4 + w) * kNearTI;
+    if (t0 >= m) return;  // the whole wave leaves together; no block barriers below
+    const uint32_t t = t0 + ti;
+    const bool tv = t < m;
+    const float4 a = epos[tv ? t : t0];
     const int gi = (int)g;
     const size_t G = (size_t)g * g * g;
-    const float4 a = epos[t];
-    const int i = cell_of(a.x, invh), j = cell_of(a.y, invh), k = cell_of(a.z, invh);
 
-    // ---- M2P: hat weights about cell centres, in-grid cells only
+    // ---- M2P: hat weights about cell centres, in-grid cells only; slice sj < 8
+    // of a point reads corner sj of its 2x2x2 stencil
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    if (i >= 0 && i < gi && j >= 0 && j < gi && k >= 0 && k < gi) {
-        double xh = (double)a.x * invh - 0.5, yh = (double)a.y * invh - 0.5, zh = (double)a.z * invh - 0.5;
-        int i0 = (int)floor(xh), j0 = (int)floor(yh), k0 = (int)floor(zh);
-        float fx = (float)(xh - i0), fy = (float)(yh - j0), fz = (float)(zh - k0);
-        float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy}, wz[2] = {1.f - fz, fz};
-#pragma unroll
-        for (int c2 = 0; c2 < 2; c2++) {
-            int kk = k0 + c2;
-            if (kk < 0 || kk >= gi) continue;
-#pragma unroll
-            for (int b2 = 0; b2 < 2; b2++) {
-                int jj = j0 + b2;
-                if (jj < 0 || jj >= gi) continue;
-#pragma unroll
-                for (int a2 = 0; a2 < 2; a2++) {
-                    int ii = i0 + a2;
-                    if (ii < 0 || ii >= gi) continue;
-                    size_t q = ((size_t)kk * g + jj) * g + ii;
-                    float w = wx[a2] * wy[b2] * wz[c2];
-                    sx += w * far[q];
-                    sy += w * far[G + q];
-                    sz += w * far[2 * G + q];
-                }
+    {
+        const int i = cell_of(a.x, invh), j = cell_of(a.y, invh), k = cell_of(a.z, invh);
+        if (tv && sj < 8 && i >= 0 && i < gi && j >= 0 && j < gi && k >= 0 && k < gi) {
+            const double xh = (double)a.x * invh - 0.5, yh = (double)a.y * invh - 0.5, zh = (double)a.z * invh - 0.5;
+            const int i0 = (int)floor(xh), j0 = (int)floor(yh), k0 = (int)floor(zh);
+            const float fx = (float)(xh - i0), fy = (float)(yh - j0), fz = (float)(zh - k0);
+            const int a2 = sj & 1, b2 = (sj >> 1) & 1, c2 = sj >> 2;
+            const int ii = i0 + a2, jj = j0 + b2, kk = k0 + c2;
+            if (ii >= 0 && ii < gi && jj >= 0 && jj < gi && kk >= 0 && kk < gi) {
+                const size_t q = ((size_t)kk * g + jj) * g + ii;
+                const float wgt = (a2 ? fx : 1.f - fx) * (b2 ? fy : 1.f - fy) * (c2 ? fz : 1.f - fz);
+                sx = wgt * far[q];
+                sy = wgt * far[G + q];
+                sz = wgt * far[2 * G + q];
             }
         }
     }
 
-    // ---- near field over the hash rows of the point's box
     float vx = 0.f, vy = 0.f, vz = 0.f;
     if (K >= 0) {
-        float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (SEGMENT) bpos = eoth[t];
-        const int R = K + 1;
-        const int xlo = max(i - R, 0), xhi = min(i + R, gi - 1);
-        const int ylo = max(j - R, 0), yhi = min(j + R, gi - 1);
-        const int zlo = max(k - R, 0), zhi = min(k + R, gi - 1);
+        // ---- union of the points' boxes -> scanned cell range (wave-uniform)
+        float mnx = tv ? a.x : 3.0e38f, mny = tv ? a.y : 3.0e38f, mnz = tv ? a.z : 3.0e38f;
+        float mxx = tv ? a.x : -3.0e38f, mxy = tv ? a.y : -3.0e38f, mxz = tv ? a.z : -3.0e38f;
+        for (int o = 1; o < kNearTI; o <<= 1) {
+            mnx = fminf(mnx, __shfl_xor(mnx, o, 64));
+            mny = fminf(mny, __shfl_xor(mny, o, 64));
+            mnz = fminf(mnz, __shfl_xor(mnz, o, 64));
+            mxx = fmaxf(mxx, __shfl_xor(mxx, o, 64));
+            mxy = fmaxf(mxy, __shfl_xor(mxy, o, 64));
+            mxz = fmaxf(mxz, __shfl_xor(mxz, o, 64));
+        }
+        const double rb = (double)nc.rbox;
+        const float rbox = nc.rbox;
+        const int xlo = __builtin_amdgcn_readfirstlane(near_lo_cell(mnx, rb, invh));
+        const int xhi = __builtin_amdgcn_readfirstlane(near_hi_cell(mxx, rb, invh, gi));
+        const int ylo = __builtin_amdgcn_readfirstlane(near_lo_cell(mny, rb, invh));
+        const int yhi = __builtin_amdgcn_readfirstlane(near_hi_cell(mxy, rb, invh, gi));
+        const int zlo = __builtin_amdgcn_readfirstlane(near_lo_cell(mnz, rb, invh));
+        const int zhi = __builtin_amdgcn_readfirstlane(near_hi_cell(mxz, rb, invh, gi));
         const uint32_t nsrc = off[G];
-        if (xlo <= xhi)
+        float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (SEGMENT) bpos = eoth[tv ? t : t0];
+        if (xlo <= xhi) {
             for (int kk = zlo; kk <= zhi; kk++) {
                 for (int jj = ylo; jj <= yhi; jj++) {
                     const uint32_t rowc = ((uint32_t)kk * g + (uint32_t)jj) * g;
-                    const uint32_t p0 = off[rowc + xlo], p1 = min(off[rowc + xhi + 1], nsrc);
-                    for (uint32_t p = p0; p < p1; p++) {
-                        const float4 s = spos[p];
-                        const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
-                        if (!(fabsf(dx) <= nc.rbox && fabsf(dy) <= nc.rbox && fabsf(dz) <= nc.rbox)) continue;
-                        const float r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 < 1e-24f) continue;
-                        if (SEGMENT) {
-                            const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
-                            if (ex * ex + ey * ey + ez * ez < 1e-24f) continue;
+                    const uint32_t p0 = __builtin_amdgcn_readfirstlane(off[rowc + xlo]);
+                    const uint32_t p1 = __builtin_amdgcn_readfirstlane(min(off[rowc + xhi + 1], nsrc));
+                    for (uint32_t base = p0; base < p1; base += 64) {
+                        const uint32_t cnt = min(64u, p1 - base);
+                        if ((uint32_t)lane < cnt) {
+                            cpos[w][lane] = spos[base + lane];
+                            comg[w][lane] = somg[base + lane];
                         }
-                        const float cf =
-                            KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc) : coeff_expcubic(r2, nc);
-                        const float4 o = somg[p];
-                        vx += (o.y * dz - o.z * dy) * cf;
-                        vy += (o.z * dx - o.x * dz) * cf;
-                        vz += (o.x * dy - o.y * dx) * cf;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        for (uint32_t q = sj; q < cnt; q += kNearSJ) {
+                            const float4 s = cpos[w][q];
+                            const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
+                            bool acc = tv && fabsf(dx) <= rbox && fabsf(dy) <= rbox && fabsf(dz) <= rbox;
+                            const float r2 = dx * dx + dy * dy + dz * dz;
+                            acc = acc && r2 >= 1e-24f;
+                            if (SEGMENT) {
+                                const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
+                                acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
+                            }
+                            if (acc) {
+                                const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
+                                                                                : coeff_expcubic(r2, nc);
+                                const float4 o = comg[w][q];
+                                vx += (o.y * dz - o.z * dy) * cf;
+                                vy += (o.z * dx - o.x * dz) * cf;
+                                vz += (o.x * dy - o.y * dx) * cf;
+                            }
+                        }
+                        // the next chunk overwrites the staging slots
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     }
                 }
             }
+        }
     }
-    const uint32_t dt = eidx[t];
-    if (dt >= m) return;
-    const size_t dst = (size_t)dt * u_stride;
-    ux[dst] = (TO)(sx + vx);
-    uy[dst] = (TO)(sy + vy);
-    uz[dst] = (TO)(sz + vz);
+    // ---- reduce over the slices of each point
+    float rx = sx + vx, ry = sy + vy, rz = sz + vz;
+    for (int o = kNearTI; o < 64; o <<= 1) {
+        rx += __shfl_xor(rx, o, 64);
+        ry += __shfl_xor(ry, o, 64);
+        rz += __shfl_xor(rz, o, 64);
+    }
+    if (sj == 0 && tv) {
+        const uint32_t dt = eidx[t];
+        if (dt < m) {
+            const size_t dst = (size_t)dt * u_stride;
+            ux[dst] = (TO)rx;
+            uy[dst] = (TO)ry;
+            uz[dst] = (TO)rz;
+        }
+    }
 }
 
 template <int KERNEL, bool SEGMENT>
 static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st) {
-    dim3 grid(div_up(a.m, 256)), blk(256);
+    const uint32_t waves = div_up(a.m, kNearTI);
+    dim3 grid(div_up(waves, 4)), blk(256);
     if (a.u_dtype == PPPM_F64)
         hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
                            a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy,

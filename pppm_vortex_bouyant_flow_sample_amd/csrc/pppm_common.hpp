@@ -30,6 +30,16 @@ constexpr double kRefDiagPsi = 0.250580605913965;
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline uint64_t div_up64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Workgroup renumbering for the 8 XCDs of an MI355X.  Consecutive workgroup
+// ids are handed to the XCDs round robin, so neighbouring tiles (which share
+// halo data) would land in 8 different L2 caches; with this renumbering XCD x
+// walks one contiguous range [x*per + min(x, rem), ...) of the n workgroups.
+__device__ inline uint32_t xcd_block(uint32_t id, uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t x = id % kXcd, loc = id / kXcd, per = n / kXcd, rem = n % kXcd;
+    return x * per + (x < rem ? x : rem) + loc;
+}
+
 // float -> int as the GPU hardware converts (round toward zero, saturating, NaN -> 0)
 __device__ inline int f2i_sat(float f) {
     if (!(f == f)) return 0;
