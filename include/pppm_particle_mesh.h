@@ -1,0 +1,64 @@
+/*
+ * pppm_particle_mesh.h -- the reference's particle-mesh free functions of the
+ * Biot-Savart path, same names and arguments (paths relative to the
+ * reference repository: hybridFastSummation/hybridFastSummation/ParticleMesh.h,
+ * kernels in GPU_kernel.cu).  FP64, device pointers, enqueued on the null
+ * stream like the reference.  Arrays "sorted" are in the hash order of
+ * start/end (ParticleManager / calcHash + sortParticles + reorder).
+ *
+ * C++ callers only (the arguments are HIP vector types).
+ */
+#ifndef PPPM_PARTICLE_MESH_H
+#define PPPM_PARTICLE_MESH_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+extern "C" {
+
+/* ParticleMesh.h:9-18, GPU_kernel.cu:218-297: gridval[c] = h^-3 * sum of mass
+ * over the sorted particles within +-2 hash cells of c whose box weight about
+ * c's centre is one. */
+void ParticleToMesh(uint32_t *start, uint32_t *end, float4 *pos, double *mass, double cell_h, double *gridval,
+                    uint3 griddim, uint3 hashdim, uint32_t num_particle, double4 origin);
+
+/* ParticleMesh.h:40, GPU_kernel.cu:524-543: gridval *= h_sqr * G. */
+void ComputeRHS(double *gridval, double h_sqr, double G, uint32_t num_cell);
+
+/* ParticleMesh.h:139-148, GPU_kernel.cu:1750-1805: (u, v, w) = curl F with
+ * central differences, indices clamped to the grid. */
+void getCurl(double *Fx, double *Fy, double *Fz, double *u, double *v, double *w, uint32_t dimx, uint32_t dimy,
+             uint32_t dimz, double h);
+
+/* ParticleMesh.h:150-157, GPU_kernel.cu:1809-1870: trilinear (hat-weighted)
+ * interpolation of mesh_value at each particle over the in-grid cells. */
+void MeshToParticle(float4 *pos, double *mesh_value, double *particle_value, double h, uint3 gdim, uint3 hdim,
+                    uint32_t num_particles, double4 origin);
+
+/* ParticleMesh.h:194-216, GPU_kernel.cu:2262-2485: near-field correction of
+ * the sorted evaluation points against the sorted vortices, added to u, v, w
+ * (in/out; u_pm, v_pm, w_pm are unused, as in the reference).  Core radius
+ * 0.08, box |d|inf < (K + 1/2) h over hash cells +-(K+1); is_segment = 1
+ * also skips vortices within 1e-12 of evalposb.  K < 0: nothing. */
+void BiotSavartPPCorrScaleMN(uint32_t *start, uint32_t *end, float4 *evalposa, float4 *evalposb, uint32_t is_segment,
+                             float4 *vortpos, double *vort_x, double *vort_y, double *vort_z, double *u, double *v,
+                             double *w, double *u_pm, double *v_pm, double *w_pm, double cellh, uint3 gdim,
+                             uint3 hdim, int K, uint32_t M, uint32_t N, double4 origin);
+
+/* ParticleMesh.h:281-286, GPU_kernel.cu:2487-2734: far-field buffer
+ * u = g_u - sum_{D != 0, |D|inf <= K} dir[D] x vrho[c+D] h^3 - curl(0.250580605913965 h^2 vrho),
+ * with g_u the grid velocity on entry.  As in the reference, psi_* and g_*
+ * are overwritten (psi = 0.2505806 h^2 vrho, g = its curl).  K < 0: nothing. */
+void BiotSavartComputeFarField(uint32_t *start, uint32_t *end, float4 *pos, double *vrho_x, double *vrho_y,
+                               double *vrho_z, double *psi_x, double *psi_y, double *psi_z, double *g_u, double *g_v,
+                               double *g_w, double *u, double *v, double *w, double cell_h, uint3 gdim, uint3 hdim,
+                               int K, uint32_t num_particles, double4 origin);
+
+/* ParticleMesh.h:288-294, GPU_kernel.cu:2886-2976: trilinear interpolation
+ * of the far-field buffer at each point whose cell is inside the grid
+ * (other points' outputs are left untouched). */
+void BiotSavartInterpolateFarField(float4 *pos, double *far_u, double *far_v, double *far_w, double *p_u, double *p_v,
+                                   double *p_w, double cell_h, uint32_t dimx, uint32_t dimy, uint32_t dimz,
+                                   uint32_t num_particles, double4 origin);
+}
+
+#endif

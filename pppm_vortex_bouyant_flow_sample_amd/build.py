@@ -27,7 +27,7 @@ FLAGS = [
 
 SOURCES = [
     "scan.hip", "hash.hip", "p2m.hip", "mg.hip", "farfield.hip", "nearfield.hip",
-    "api.hip", "objects.hip", "ref_abi.hip", "serve.hip",
+    "api.hip", "objects.hip", "ref_abi.hip", "serve.hip", "particle_mesh.hip",
 ]
 
 
@@ -64,24 +64,31 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+DROPIN_PROGRAMS = ["dropin_main", "pm_main"]
+
+
 def build_dropin_test():
-    """The C++ program that drives the drop-in classes (tests/dropin), built
-    with the host compiler against include/ and the library."""
-    src = os.path.join(ROOT, "tests", "dropin", "dropin_main.cpp")
-    if not os.path.exists(src):
-        return None
+    """The C++ programs that drive the drop-in classes and functions
+    (tests/dropin), built with the host compiler against include/ and the
+    library, as a caller of the reference would build them."""
     out_dir = os.path.join(ROOT, "tests", "dropin", "_build")
     os.makedirs(out_dir, exist_ok=True)
-    exe = os.path.join(out_dir, "dropin_main")
-    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           "-I", os.path.join(ROOT, "include"), src, "-o", exe + ".tmp",
-           "-L", OUT_DIR, "-lpppm_hip", "-Wl,-rpath," + OUT_DIR,
-           "-Wl,-rpath,$ORIGIN/../../../pppm_vortex_bouyant_flow_sample_amd/_build"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("drop-in test program failed to build:\n" + r.stdout + r.stderr)
-    os.replace(exe + ".tmp", exe)
-    return exe
+    built = []
+    for name in DROPIN_PROGRAMS:
+        src = os.path.join(ROOT, "tests", "dropin", name + ".cpp")
+        if not os.path.exists(src):
+            continue
+        exe = os.path.join(out_dir, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+               "-I", os.path.join(ROOT, "include"), src, "-o", exe + ".tmp",
+               "-L", OUT_DIR, "-lpppm_hip", "-Wl,-rpath," + OUT_DIR,
+               "-Wl,-rpath,$ORIGIN/../../../pppm_vortex_bouyant_flow_sample_amd/_build"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"drop-in test program {name} failed to build:\n" + r.stdout + r.stderr)
+        os.replace(exe + ".tmp", exe)
+        built.append(exe)
+    return built
 
 
 if __name__ == "__main__":

@@ -17,73 +17,125 @@ __device__ inline double ld(const TP* p, size_t i) {
     return (double)p[i];
 }
 
-template <typename TP, typename TC>
+// A workgroup owns a 32 x 8 tile of cells and a run of z planes.  The grid
+// vorticity streams through an LDS ring of 2K+1 planes with a K-cell halo
+// (zero outside the grid, which is the reference's in-grid-only sum), so the
+// (2K+1)^3 local correction reads LDS instead of global memory; the curl of
+// psi and of the diagonal term read global memory (12 + 12 loads).
+constexpr int kCfTX = 32, kCfTY = 8, kCfZC = 16;
+
+template <typename TP, typename TC, int K>
 __global__ void __launch_bounds__(256) k_curl_far(const TP* __restrict__ psi, const float* __restrict__ wg,
                                                   float* __restrict__ far, const float* __restrict__ dir, uint32_t g,
-                                                  int K, float h3, TC half_invh, float cdiag_half_invh) {
+                                                  float h3, TC half_invh, float cdiag_half_invh) {
+    constexpr int W = 2 * K + 1, LXW = kCfTX + 2 * K, LYW = kCfTY + 2 * K, PLW = LXW * LYW;
+    __shared__ float ring[W > 0 ? W : 1][3][PLW];
     const uint32_t G = g * g * g;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= G) return;
     const int gi = (int)g;
-    const int i = (int)(c % g), j = (int)((c / g) % g), k = (int)(c / (g * g));
-    const int im = i > 0 ? i - 1 : 0, ip = i < gi - 1 ? i + 1 : gi - 1;
-    const int jm = j > 0 ? j - 1 : 0, jp = j < gi - 1 ? j + 1 : gi - 1;
-    const int km = k > 0 ? k - 1 : 0, kp = k < gi - 1 ? k + 1 : gi - 1;
+    const int tilesx = (gi + kCfTX - 1) / kCfTX, tilesy = (gi + kCfTY - 1) / kCfTY;
+    const int bx = blockIdx.x % tilesx, by = blockIdx.x / tilesx;
+    const int z0 = blockIdx.y * kCfZC, z1 = min(z0 + kCfZC, gi);
+    const int lx = threadIdx.x % kCfTX, ly = threadIdx.x / kCfTX;
+    const int i = bx * kCfTX + lx, j = by * kCfTY + ly;
+    const int gx0 = bx * kCfTX - K, gy0 = by * kCfTY - K;
     auto at = [&](int a, int b, int cc) { return ((size_t)cc * g + (size_t)b) * g + (size_t)a; };
-    const TP* Px = psi;
-    const TP* Py = psi + G;
-    const TP* Pz = psi + 2 * (size_t)G;
-    // curl(psi), GPU_kernel.cu:1770-1786 (evaluation order kept)
-    TC ux = half_invh * ((TC)Pz[at(i, jp, k)] - (TC)Pz[at(i, jm, k)] + (TC)Py[at(i, j, km)] - (TC)Py[at(i, j, kp)]);
-    TC uy = half_invh * ((TC)Px[at(i, j, kp)] - (TC)Px[at(i, j, km)] + (TC)Pz[at(im, j, k)] - (TC)Pz[at(ip, j, k)]);
-    TC uz = half_invh * ((TC)Py[at(ip, j, k)] - (TC)Py[at(im, j, k)] + (TC)Px[at(i, jm, k)] - (TC)Px[at(i, jp, k)]);
     const float* Wx = wg;
     const float* Wy = wg + G;
     const float* Wz = wg + 2 * (size_t)G;
-    // diagonal term: curl of 0.2505806 h^2 w with the same clamped stencil
-    float dx = cdiag_half_invh * (Wz[at(i, jp, k)] - Wz[at(i, jm, k)] + Wy[at(i, j, km)] - Wy[at(i, j, kp)]);
-    float dy = cdiag_half_invh * (Wx[at(i, j, kp)] - Wx[at(i, j, km)] + Wz[at(im, j, k)] - Wz[at(ip, j, k)]);
-    float dz = cdiag_half_invh * (Wy[at(ip, j, k)] - Wy[at(im, j, k)] + Wx[at(i, jm, k)] - Wx[at(i, jp, k)]);
-    // local correction over the (2K+1)^3 block
-    const int W = 2 * K + 1;
-    float vx = 0.f, vy = 0.f, vz = 0.f;
-    for (int a = -K; a <= K; a++) {
-        const int kk = k + a;
-        if (kk < 0 || kk >= gi) continue;
-        for (int b = -K; b <= K; b++) {
-            const int jj = j + b;
-            if (jj < 0 || jj >= gi) continue;
-            const float* drow = dir + 3 * (size_t)(((a + K) * W + (b + K)) * W);
-            for (int e = -K; e <= K; e++) {
-                const int ii = i + e;
-                if (ii < 0 || ii >= gi) continue;
-                const float d0 = drow[3 * (e + K)], d1 = drow[3 * (e + K) + 1], d2 = drow[3 * (e + K) + 2];
-                const size_t q = at(ii, jj, kk);
-                const float ox = Wx[q] * h3, oy = Wy[q] * h3, oz = Wz[q] * h3;
-                vx += d1 * oz - d2 * oy;
-                vy += d2 * ox - d0 * oz;
-                vz += d0 * oy - d1 * ox;
+    auto load_plane = [&](int kk) {
+        float* rp = &ring[((kk % W) + W) % W][0][0];
+        for (int q = threadIdx.x; q < PLW; q += 256) {
+            const int yy = q / LXW, xx = q - yy * LXW;
+            const int ii = gx0 + xx, jj = gy0 + yy;
+            float vx = 0.f, vy = 0.f, vz = 0.f;
+            if (kk >= 0 && kk < gi && ii >= 0 && ii < gi && jj >= 0 && jj < gi) {
+                const size_t t = at(ii, jj, kk);
+                vx = Wx[t];
+                vy = Wy[t];
+                vz = Wz[t];
             }
+            rp[q] = vx;
+            rp[PLW + q] = vy;
+            rp[2 * PLW + q] = vz;
         }
+    };
+    if (K >= 0)
+        for (int kk = z0 - K; kk < z0 + K; kk++) load_plane(kk);
+    for (int k = z0; k < z1; k++) {
+        if (K >= 0) load_plane(k + K);
+        __syncthreads();
+        if (i < gi && j < gi) {
+            const int im = i > 0 ? i - 1 : 0, ip = i < gi - 1 ? i + 1 : gi - 1;
+            const int jm = j > 0 ? j - 1 : 0, jp = j < gi - 1 ? j + 1 : gi - 1;
+            const int km = k > 0 ? k - 1 : 0, kp = k < gi - 1 ? k + 1 : gi - 1;
+            const TP* Px = psi;
+            const TP* Py = psi + G;
+            const TP* Pz = psi + 2 * (size_t)G;
+            // curl(psi), GPU_kernel.cu:1770-1786 (evaluation order kept)
+            TC ux = half_invh * ((TC)Pz[at(i, jp, k)] - (TC)Pz[at(i, jm, k)] + (TC)Py[at(i, j, km)] -
+                                 (TC)Py[at(i, j, kp)]);
+            TC uy = half_invh * ((TC)Px[at(i, j, kp)] - (TC)Px[at(i, j, km)] + (TC)Pz[at(im, j, k)] -
+                                 (TC)Pz[at(ip, j, k)]);
+            TC uz = half_invh * ((TC)Py[at(ip, j, k)] - (TC)Py[at(im, j, k)] + (TC)Px[at(i, jm, k)] -
+                                 (TC)Px[at(i, jp, k)]);
+            // diagonal term: curl of 0.2505806 h^2 w with the same clamped stencil
+            float dx = cdiag_half_invh * (Wz[at(i, jp, k)] - Wz[at(i, jm, k)] + Wy[at(i, j, km)] - Wy[at(i, j, kp)]);
+            float dy = cdiag_half_invh * (Wx[at(i, j, kp)] - Wx[at(i, j, km)] + Wz[at(im, j, k)] - Wz[at(ip, j, k)]);
+            float dz = cdiag_half_invh * (Wy[at(ip, j, k)] - Wy[at(im, j, k)] + Wx[at(i, jm, k)] - Wx[at(i, jp, k)]);
+            // local correction over the (2K+1)^3 block, same order as the reference loop
+            float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+            for (int a = -K; a <= K; a++) {
+                const float* rp = &ring[(((k + a) % W) + W) % W][0][0];
+#pragma unroll
+                for (int b = -K; b <= K; b++) {
+                    const float* drow = dir + 3 * (size_t)(((a + K) * W + (b + K)) * W);
+                    const int row = (ly + K + b) * LXW + lx + K;
+#pragma unroll
+                    for (int e = -K; e <= K; e++) {
+                        if (a == 0 && b == 0 && e == 0) continue;
+                        const float d0 = drow[3 * (e + K)], d1 = drow[3 * (e + K) + 1], d2 = drow[3 * (e + K) + 2];
+                        const float ox = rp[row + e] * h3, oy = rp[PLW + row + e] * h3, oz = rp[2 * PLW + row + e] * h3;
+                        vx += d1 * oz - d2 * oy;
+                        vy += d2 * ox - d0 * oz;
+                        vz += d0 * oy - d1 * ox;
+                    }
+                }
+            }
+            const size_t c = at(i, j, k);
+            far[c] = (float)((double)ux - (double)vx - (double)dx);
+            far[(size_t)G + c] = (float)((double)uy - (double)vy - (double)dy);
+            far[2 * (size_t)G + c] = (float)((double)uz - (double)vz - (double)dz);
+        }
+        __syncthreads();  // the next plane load reuses the slot of plane k - K
     }
-    far[c] = (float)((double)ux - (double)vx - (double)dx);
-    far[(size_t)G + c] = (float)((double)uy - (double)vy - (double)dy);
-    far[2 * (size_t)G + c] = (float)((double)uz - (double)vz - (double)dz);
+}
+
+template <typename TP, typename TC>
+static void launch_curl_far_k(const TP* psi, const float* wgrid, float* far, const float* dir, uint32_t g, int K,
+                              float h3, TC half_invh, float cd, hipStream_t st) {
+    const uint32_t tiles = div_up(g, kCfTX) * div_up(g, kCfTY);
+    dim3 grid(tiles, div_up(g, kCfZC)), blk(256);
+    switch (K) {
+        case 0: hipLaunchKernelGGL((k_curl_far<TP, TC, 0>), grid, blk, 0, st, psi, wgrid, far, dir, g, h3, half_invh, cd); break;
+        case 1: hipLaunchKernelGGL((k_curl_far<TP, TC, 1>), grid, blk, 0, st, psi, wgrid, far, dir, g, h3, half_invh, cd); break;
+        case 2: hipLaunchKernelGGL((k_curl_far<TP, TC, 2>), grid, blk, 0, st, psi, wgrid, far, dir, g, h3, half_invh, cd); break;
+        case 3: hipLaunchKernelGGL((k_curl_far<TP, TC, 3>), grid, blk, 0, st, psi, wgrid, far, dir, g, h3, half_invh, cd); break;
+        case 4: hipLaunchKernelGGL((k_curl_far<TP, TC, 4>), grid, blk, 0, st, psi, wgrid, far, dir, g, h3, half_invh, cd); break;
+        default: hipLaunchKernelGGL((k_curl_far<TP, TC, -1>), grid, blk, 0, st, psi, wgrid, far, dir, g, h3, half_invh, cd); break;
+    }
 }
 
 pppm_status launch_curl_far_impl(const void* psi, int psi_dtype, const float* wgrid, float* far, const float* dir,
                                  uint32_t g, int K, double h, hipStream_t st) {
-    const uint32_t G = g * g * g;
     const float h3 = (float)(h * h * h);
     const double half_invh = 0.5 * (1.0 / h);
     // vortex-in-cell (K < 0): the far-field buffer is the grid velocity itself
     const float cd = K < 0 ? 0.0f : (float)(kRefDiagPsi * h * h * half_invh);
     if (psi_dtype == PPPM_F64)
-        hipLaunchKernelGGL((k_curl_far<double, double>), dim3(div_up(G, 256)), dim3(256), 0, st, (const double*)psi,
-                           wgrid, far, dir, g, K, h3, half_invh, cd);
+        launch_curl_far_k<double, double>((const double*)psi, wgrid, far, dir, g, K, h3, half_invh, cd, st);
     else
-        hipLaunchKernelGGL((k_curl_far<float, float>), dim3(div_up(G, 256)), dim3(256), 0, st, (const float*)psi,
-                           wgrid, far, dir, g, K, h3, (float)half_invh, cd);
+        launch_curl_far_k<float, float>((const float*)psi, wgrid, far, dir, g, K, h3, (float)half_invh, cd, st);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }

@@ -885,6 +885,18 @@ pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float b
 }
 
 template <typename T>
+pppm_status MGHier<T>::smooth_colour(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st) {
+    const int nx = d_[l][0], ny = d_[l][1], nz = d_[l][2];
+    dim3 blk(64, 4), grid(div_up(div_up(nx, 2), 64), div_up(ny, 4), nz * ncomp_);
+    for (int it = 0; it < iters; it++)
+        for (int color = 1; color >= 0; color--) {
+            hipLaunchKernelGGL(k_rbgs<T>, grid, blk, 0, st, x, b, bscale, nx, ny, nz, cells(l), color, omega, beta);
+            PPPM_LAUNCH_CHECK();
+        }
+    return PPPM_OK;
+}
+
+template <typename T>
 pppm_status MGHier<T>::smooth(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st) {
     const int nx = d_[l][0], ny = d_[l][1], nz = d_[l][2];
     if (std::is_same<T, float>::value && tb_ && iters > 0) {
@@ -1001,10 +1013,22 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
         a.Gc = m.cells(l + 1);
         return a;
     };
+    // Blocked passes pay a fixed latency per plane that small levels cannot
+    // amortise: levels with fewer than 2M cells run the one-launch-per-colour
+    // kernels in place instead.
+    auto blocked = [&](int l) { return m.cells(l) >= ((size_t)1 << 21); };
     // down: nu1 passes per level, x from X (or zero below the finest level)
     // ping-ponging with the level's temporary; the last pass also restricts
     std::vector<float*> cur(L, nullptr);
     for (int l = 0; l < L - 1; l++) {
+        if (!blocked(l)) {
+            const float bs = l == 0 ? bscale0 : 1.0f, beta = (float)mg_beta(l);
+            if (l > 0 && (s = m.fill(X[l], m.cells(l) * nc, st)) != PPPM_OK) return s;
+            if ((s = m.smooth_colour(l, X[l], B[l], bs, nu1, omega, beta, st)) != PPPM_OK) return s;
+            if ((s = m.residual_restrict(l, X[l], B[l], bs, beta, B[l + 1], st)) != PPPM_OK) return s;
+            cur[l] = X[l];
+            continue;
+        }
         float* src = X[l];
         float* dst = m.temp(l);
         for (int q = 0; q < nu1; q++) {
@@ -1023,6 +1047,13 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
     if ((s = m.coarse(X[L - 1], B[L - 1], L == 1 ? bscale0 : 1.0f, true, st)) != PPPM_OK) return s;
     // up: nu2 passes per level, the first adds the prolonged correction on load
     for (int l = L - 2; l >= 0; l--) {
+        if (!blocked(l)) {
+            const float bs = l == 0 ? bscale0 : 1.0f, beta = (float)mg_beta(l);
+            if ((s = m.prolong(l, X[l], X[l + 1], PROLONG_TRILINEAR, (float)mg_beta(l + 1), st)) != PPPM_OK)
+                return s;
+            if ((s = m.smooth_colour(l, X[l], B[l], bs, nu2, omega, beta, st)) != PPPM_OK) return s;
+            continue;
+        }
         float* src = cur[l];
         float* dst = src == X[l] ? m.temp(l) : X[l];
         for (int q = 0; q < nu2; q++) {
@@ -1043,8 +1074,9 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
 template <typename T>
 pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent,
                               hipStream_t st) {
-    bool even = true;  // the fused prolongation takes parents as i / 2
-    for (int l = 0; l + 1 < L_; l++) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
+    bool even = true;  // the blocked prolongation takes parents as i / 2
+    for (int l = 0; l + 1 < L_; l++)
+        if (cells(l) >= ((size_t)1 << 21)) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
     if (std::is_same<T, float>::value && consistent && tb_ && (nu1 == 1 || nu1 == 2) && (nu2 == 1 || nu2 == 2) &&
         L_ > 1 && even) {
         std::vector<T*> X = x_, B = b_;

@@ -51,6 +51,8 @@ class MGHier {
 
     // raw pieces (also used by the reference-named entry points)
     pppm_status smooth(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st);
+    // red-black iterations with one launch per colour, in place
+    pppm_status smooth_colour(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st);
     pppm_status residual_restrict(int l, const T* x, const T* b, T bscale, T beta, T* bnext, hipStream_t st);
     pppm_status prolong(int l, T* x, const T* xc, int mode, T betac, hipStream_t st);
     pppm_status coarse(T* x, const T* b, T bscale, bool consistent, hipStream_t st);

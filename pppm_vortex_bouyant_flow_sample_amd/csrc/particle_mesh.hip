@@ -1,0 +1,354 @@
+// The reference's particle-mesh free functions for the Biot-Savart path
+// (hybridFastSummation/hybridFastSummation/ParticleMesh.h:9-18, 40, 139-157,
+// 194-216, 281-294; kernels in GPU_kernel.cu), in FP64 on the null stream as
+// the reference runs them.  They are the building blocks a caller of the
+// reference orchestrates itself (BiotSavartSolverImp.cpp:517-816); the
+// context API (pppm.h) runs the same path with its own fused kernels.
+#include <vector>
+
+#include "pppm_internal.hpp"
+#include "../../include/pppm_particle_mesh.h"
+
+namespace pppm {
+namespace {
+
+__device__ inline float shifted(float p, double o) { return __double2float_rn((double)p - o); }
+
+// box weight of the deposit (GPU_kernel.cu:31-50)
+__device__ inline double box_weight(double r) { return (r < 0.5 && r >= -0.5) ? 1.0 : 0.0; }
+// hat weight of the interpolation (GPU_kernel.cu:51-84)
+__device__ inline double hat_weight(double r) {
+    const double a = fabs(r);
+    return a < 1.0 ? 1.0 - a : 0.0;
+}
+
+// GPU_kernel.cu:218-271: grid cell (i, j, k) gathers the vortices of the hash
+// cells within +-2 whose box weight about its centre is one.
+__global__ void __launch_bounds__(256) k_pm_p2m(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                                const float4* __restrict__ pos, const double* __restrict__ mass,
+                                                double invh, double* __restrict__ grid, uint3 gd, uint3 hd,
+                                                uint32_t n, double ox, double oy, double oz) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t G = gd.x * gd.y * gd.z;
+    if (c >= G) return;
+    const int i = (int)(c % gd.x), j = (int)((c / gd.x) % gd.y), k = (int)(c / (gd.x * gd.y));
+    const double ci = (double)(float)(i + 0.5), cj = (double)(float)(j + 0.5), ck = (double)(float)(k + 0.5);
+    double sum = 0.0;
+    for (int kk = k - 2; kk <= k + 2; kk++)
+        for (int jj = j - 2; jj <= j + 2; jj++)
+            for (int ii = i - 2; ii <= i + 2; ii++) {
+                if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x) continue;
+                const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                for (uint32_t p = start[q]; p < end[q]; p++) {
+                    if (p >= n) continue;
+                    const float4 s = pos[p];
+                    const double w = box_weight((double)shifted(s.x, ox) * invh - ci) *
+                                     box_weight((double)shifted(s.y, oy) * invh - cj) *
+                                     box_weight((double)shifted(s.z, oz) * invh - ck);
+                    sum += w * mass[p];
+                }
+            }
+    grid[c] = invh * invh * invh * sum;
+}
+
+__global__ void k_pm_scale(double* __restrict__ a, uint32_t n, double s) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) a[c] = a[c] * s;
+}
+
+__device__ inline double clamped(const double* f, int x, int y, int z, int nx, int ny, int nz) {
+    x = min(max(x, 0), nx - 1);
+    y = min(max(y, 0), ny - 1);
+    z = min(max(z, 0), nz - 1);
+    return f[((size_t)z * ny + y) * nx + x];
+}
+
+// GPU_kernel.cu:1750-1789
+__global__ void __launch_bounds__(256) k_pm_curl(const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                                 const double* __restrict__ Fz, double* __restrict__ u,
+                                                 double* __restrict__ v, double* __restrict__ w, int nx, int ny,
+                                                 int nz, double inv_h) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)nx * ny * nz) return;
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    const double Fz_up = clamped(Fz, i, j + 1, k, nx, ny, nz), Fz_down = clamped(Fz, i, j - 1, k, nx, ny, nz);
+    const double Fz_left = clamped(Fz, i - 1, j, k, nx, ny, nz), Fz_right = clamped(Fz, i + 1, j, k, nx, ny, nz);
+    const double Fy_front = clamped(Fy, i, j, k - 1, nx, ny, nz), Fy_back = clamped(Fy, i, j, k + 1, nx, ny, nz);
+    const double Fy_left = clamped(Fy, i - 1, j, k, nx, ny, nz), Fy_right = clamped(Fy, i + 1, j, k, nx, ny, nz);
+    const double Fx_front = clamped(Fx, i, j, k - 1, nx, ny, nz), Fx_back = clamped(Fx, i, j, k + 1, nx, ny, nz);
+    const double Fx_up = clamped(Fx, i, j + 1, k, nx, ny, nz), Fx_down = clamped(Fx, i, j - 1, k, nx, ny, nz);
+    u[c] = 0.5 * inv_h * (Fz_up - Fz_down + Fy_front - Fy_back);
+    v[c] = 0.5 * inv_h * (Fx_back - Fx_front + Fz_left - Fz_right);
+    w[c] = 0.5 * inv_h * (Fy_right - Fy_left + Fx_down - Fx_up);
+}
+
+// GPU_kernel.cu:1809-1852: hat-weighted sum over the in-grid cells within +-2
+__global__ void __launch_bounds__(256) k_pm_m2p(const float4* __restrict__ pos, const double* __restrict__ mesh,
+                                                double* __restrict__ out, double invh, uint3 gd, uint32_t n,
+                                                double ox, double oy, double oz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 p = pos[t];
+    const float px = shifted(p.x, ox), py = shifted(p.y, oy), pz = shifted(p.z, oz);
+    const int i = (int)floor(px * invh), j = (int)floor(py * invh), k = (int)floor(pz * invh);
+    double sum = 0.0;
+    for (int kk = k - 2; kk <= k + 2; kk++)
+        for (int jj = j - 2; jj <= j + 2; jj++)
+            for (int ii = i - 2; ii <= i + 2; ii++) {
+                if (kk < 0 || kk >= (int)gd.z || jj < 0 || jj >= (int)gd.y || ii < 0 || ii >= (int)gd.x) continue;
+                const double f = mesh[((size_t)kk * gd.y + jj) * gd.x + ii];
+                const double w = hat_weight((double)px * invh - (double)(float)(ii + 0.5)) *
+                                 hat_weight((double)py * invh - (double)(float)(jj + 0.5)) *
+                                 hat_weight((double)pz * invh - (double)(float)(kk + 0.5));
+                sum += f * w;
+            }
+    out[t] = sum;
+}
+
+// GPU_kernel.cu:2886-2964: as above for three components, only for points
+// whose cell is inside the grid (others are left untouched)
+__global__ void __launch_bounds__(256) k_pm_interp_far(const float4* __restrict__ pos, const double* __restrict__ fu,
+                                                       const double* __restrict__ fv, const double* __restrict__ fw,
+                                                       double* __restrict__ pu, double* __restrict__ pv,
+                                                       double* __restrict__ pw, double invh, int nx, int ny, int nz,
+                                                       uint32_t n, double ox, double oy, double oz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 p = pos[t];
+    const float px = shifted(p.x, ox), py = shifted(p.y, oy), pz = shifted(p.z, oz);
+    const int i = (int)floor(px * invh), j = (int)floor(py * invh), k = (int)floor(pz * invh);
+    if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) return;
+    double sx = 0, sy = 0, sz = 0;
+    for (int kk = k - 2; kk <= k + 2; kk++)
+        for (int jj = j - 2; jj <= j + 2; jj++)
+            for (int ii = i - 2; ii <= i + 2; ii++) {
+                if (kk < 0 || kk >= nz || jj < 0 || jj >= ny || ii < 0 || ii >= nx) continue;
+                const size_t q = ((size_t)kk * ny + jj) * nx + ii;
+                const double w = hat_weight((double)(float)(ii + 0.5) - (double)px * invh) *
+                                 hat_weight((double)(float)(jj + 0.5) - (double)py * invh) *
+                                 hat_weight((double)(float)(kk + 0.5) - (double)pz * invh);
+                sx += fu[q] * w;
+                sy += fv[q] * w;
+                sz += fw[q] * w;
+            }
+    pu[t] = sx;
+    pv[t] = sy;
+    pw[t] = sz;
+}
+
+// GPU_kernel.cu:2262-2436 (BiotSavartPPCorrScaleMN_kernel): u += sum over the
+// vortices of the hash cells within +-(K+1) that lie inside the box
+// |d|inf < (K + 1/2) h of (w x d) q(r) / (4 pi r^3), q = 1 - exp(-(r/a)^3),
+// a = 0.08; coincident vortices (r < 1e-12, and in segment mode also the
+// other end) contribute nothing.
+__global__ void __launch_bounds__(256) k_pm_near(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                                                 const float4* __restrict__ pa, const float4* __restrict__ pb,
+                                                 uint32_t segment, const float4* __restrict__ vpos,
+                                                 const double* __restrict__ wx, const double* __restrict__ wy,
+                                                 const double* __restrict__ wz, double* __restrict__ u,
+                                                 double* __restrict__ v, double* __restrict__ w, double invh,
+                                                 uint3 hd, int K, uint32_t M, uint32_t N, double ox, double oy,
+                                                 double oz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const double R = ((double)K + 0.5) / invh, a = 0.08;
+    const float4 A = pa[t];
+    const float ax = shifted(A.x, ox), ay = shifted(A.y, oy), az = shifted(A.z, oz);
+    float bx = 0.f, by = 0.f, bz = 0.f;
+    if (segment == 1) {
+        const float4 B = pb[t];
+        bx = shifted(B.x, ox);
+        by = shifted(B.y, oy);
+        bz = shifted(B.z, oz);
+    }
+    const int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+    double sx = 0, sy = 0, sz = 0;
+    for (int kk = k - K - 1; kk <= k + K + 1; kk++)
+        for (int jj = j - K - 1; jj <= j + K + 1; jj++)
+            for (int ii = i - K - 1; ii <= i + K + 1; ii++) {
+                if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x) continue;
+                const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                for (uint32_t p = start[q]; p < end[q]; p++) {
+                    if (p >= N) continue;
+                    const float4 s = vpos[p];
+                    const float sxp = shifted(s.x, ox), syp = shifted(s.y, oy), szp = shifted(s.z, oz);
+                    const float dxf = ax - sxp, dyf = ay - syp, dzf = az - szp;
+                    if (!(fabs((double)dxf) < R && fabs((double)dyf) < R && fabs((double)dzf) < R)) continue;
+                    const double dx = dxf, dy = dyf, dz = dzf;
+                    const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (r1 < 1e-12) continue;
+                    if (segment == 1) {
+                        const double ex = (double)(bx - sxp), ey = (double)(by - syp), ez = (double)(bz - szp);
+                        if (sqrt(ex * ex + ey * ey + ez * ez) < 1e-12) continue;
+                    }
+                    const double rho = r1 / a, x3 = rho * rho * rho;
+                    const double q3 = x3 < 1e-3 ? x3 * (1.0 - x3 * (0.5 - x3 * (1.0 / 6.0 - x3 / 24.0)))
+                                                : -expm1(-x3);
+                    const double cf = q3 / (kRefFourPi * r1 * r1 * r1);
+                    const double ox_ = wx[p], oy_ = wy[p], oz_ = wz[p];
+                    sx += (oy_ * dz - oz_ * dy) * cf;
+                    sy += (oz_ * dx - ox_ * dz) * cf;
+                    sz += (ox_ * dy - oy_ * dx) * cf;
+                }
+            }
+    u[t] += sx;
+    v[t] += sy;
+    w[t] += sz;
+}
+
+// GPU_kernel.cu:2487-2564 (Step1): u = U - sum_{D != 0, in grid} dir[D] x w[c+D] h^3
+__global__ void __launch_bounds__(256) k_pm_far_step1(const double* __restrict__ gx, const double* __restrict__ gy,
+                                                      const double* __restrict__ gz, const double* __restrict__ Ux,
+                                                      const double* __restrict__ Uy, const double* __restrict__ Uz,
+                                                      double* __restrict__ u, double* __restrict__ v,
+                                                      double* __restrict__ w, const double* __restrict__ dir, int K,
+                                                      int nx, int ny, int nz, double h) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)nx * ny * nz) return;
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    const int W = 2 * K + 1;
+    const double h3 = h * h * h;
+    double sx = 0, sy = 0, sz = 0;
+    for (int kk = k - K; kk <= k + K; kk++)
+        for (int jj = j - K; jj <= j + K; jj++)
+            for (int ii = i - K; ii <= i + K; ii++) {
+                if (kk < 0 || kk >= nz || jj < 0 || jj >= ny || ii < 0 || ii >= nx) continue;
+                if (kk == k && jj == j && ii == i) continue;
+                const size_t q = ((size_t)kk * ny + jj) * nx + ii;
+                const double ox = gx[q] * h3, oy = gy[q] * h3, oz = gz[q] * h3;
+                const double* d = dir + 3 * (size_t)(((kk - k + K) * W + (jj - j + K)) * W + (ii - i + K));
+                sx += d[1] * oz - d[2] * oy;
+                sy += d[2] * ox - d[0] * oz;
+                sz += d[0] * oy - d[1] * ox;
+            }
+    u[c] = Ux[c] - sx;
+    v[c] = Uy[c] - sy;
+    w[c] = Uz[c] - sz;
+}
+
+// GPU_kernel.cu:1976-2001 then 2566-2630: psi = 0.2505806 h^2 w; U = curl psi; u -= U
+__global__ void k_pm_diag_psi(const double* __restrict__ g, double* __restrict__ psi, size_t n, double s) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) psi[c] = kRefDiagPsi * g[c] * s;
+}
+__global__ void k_pm_sub(double* __restrict__ a, const double* __restrict__ b, size_t n) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) a[c] -= b[c];
+}
+
+// dir table of the far-field correction (GPU_kernel.cu:2646-2677), FP64:
+// central difference at the target centre X = h/2 of 1/(4 pi |X - X'|) for
+// the source centre X' = (D + 1/2) h; zero for D = 0.
+std::vector<double> dir_table64(int K, double h) {
+    const int W = 2 * K + 1;
+    std::vector<double> out((size_t)W * W * W * 3, 0.0);
+    auto R = [](double ax, double ay, double az, double bx, double by, double bz) {
+        double r2 = (ax - bx) * (ax - bx) + (ay - by) * (ay - by) + (az - bz) * (az - bz);
+        return r2 > 1e-12 ? kRefFourPi * std::sqrt(r2) : 1e24;
+    };
+    for (int kk = -K; kk <= K; kk++)
+        for (int jj = -K; jj <= K; jj++)
+            for (int ii = -K; ii <= K; ii++) {
+                if (ii == 0 && jj == 0 && kk == 0) continue;
+                const double px = (ii + 0.5) * h, py = (jj + 0.5) * h, pz = (kk + 0.5) * h, X = 0.5 * h;
+                const size_t o = 3 * (size_t)(((kk + K) * W + (jj + K)) * W + (ii + K));
+                out[o] = 0.5 * ((1.0 / R(X + h, X, X, px, py, pz)) - (1.0 / R(X - h, X, X, px, py, pz))) / h;
+                out[o + 1] = 0.5 * ((1.0 / R(X, X + h, X, px, py, pz)) - (1.0 / R(X, X - h, X, px, py, pz))) / h;
+                out[o + 2] = 0.5 * ((1.0 / R(X, X, X + h, px, py, pz)) - (1.0 / R(X, X, X - h, px, py, pz))) / h;
+            }
+    return out;
+}
+
+inline uint32_t blocks(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+}  // namespace
+}  // namespace pppm
+
+using namespace pppm;
+
+extern "C" void ParticleToMesh(uint32_t* start, uint32_t* end, float4* pos, double* mass, double cell_h,
+                               double* gridval, uint3 griddim, uint3 hashdim, uint32_t num_particle, double4 origin) {
+    const size_t G = (size_t)griddim.x * griddim.y * griddim.z;
+    if (G == 0) return;
+    hipLaunchKernelGGL(k_pm_p2m, dim3(blocks(G)), dim3(256), 0, 0, start, end, pos, mass, 1.0 / cell_h, gridval,
+                       griddim, hashdim, num_particle, origin.x, origin.y, origin.z);
+}
+
+extern "C" void ComputeRHS(double* gridval, double h_sqr, double G, uint32_t num_cell) {
+    if (num_cell == 0) return;
+    hipLaunchKernelGGL(k_pm_scale, dim3(blocks(num_cell)), dim3(256), 0, 0, gridval, num_cell, h_sqr * G);
+}
+
+extern "C" void getCurl(double* Fx, double* Fy, double* Fz, double* u, double* v, double* w, uint32_t dimx,
+                        uint32_t dimy, uint32_t dimz, double h) {
+    const size_t G = (size_t)dimx * dimy * dimz;
+    if (G == 0) return;
+    hipLaunchKernelGGL(k_pm_curl, dim3(blocks(G)), dim3(256), 0, 0, Fx, Fy, Fz, u, v, w, (int)dimx, (int)dimy,
+                       (int)dimz, 1.0 / h);
+}
+
+extern "C" void MeshToParticle(float4* pos, double* mesh_value, double* particle_value, double h, uint3 gdim,
+                               uint3 hdim, uint32_t num_particles, double4 origin) {
+    (void)hdim;
+    if (num_particles == 0) return;
+    hipLaunchKernelGGL(k_pm_m2p, dim3(blocks(num_particles)), dim3(256), 0, 0, pos, mesh_value, particle_value,
+                       1.0 / h, gdim, num_particles, origin.x, origin.y, origin.z);
+}
+
+extern "C" void BiotSavartPPCorrScaleMN(uint32_t* start, uint32_t* end, float4* evalposa, float4* evalposb,
+                                        uint32_t is_segment, float4* vortpos, double* vort_x, double* vort_y,
+                                        double* vort_z, double* u, double* v, double* w, double* u_pm, double* v_pm,
+                                        double* w_pm, double cellh, uint3 gdim, uint3 hdim, int K, uint32_t M,
+                                        uint32_t N, double4 origin) {
+    (void)u_pm;
+    (void)v_pm;
+    (void)w_pm;
+    (void)gdim;
+    if (K < 0 || M == 0) return;
+    hipLaunchKernelGGL(k_pm_near, dim3(blocks(M)), dim3(256), 0, 0, start, end, evalposa, evalposb, is_segment,
+                       vortpos, vort_x, vort_y, vort_z, u, v, w, 1.0 / cellh, hdim, K, M, N, origin.x, origin.y,
+                       origin.z);
+}
+
+extern "C" void BiotSavartComputeFarField(uint32_t* start, uint32_t* end, float4* pos, double* vrho_x, double* vrho_y,
+                                          double* vrho_z, double* psi_x, double* psi_y, double* psi_z, double* g_u,
+                                          double* g_v, double* g_w, double* u, double* v, double* w, double cell_h,
+                                          uint3 gdim, uint3 hdim, int K, uint32_t num_particles, double4 origin) {
+    (void)start;
+    (void)end;
+    (void)pos;
+    (void)hdim;
+    (void)num_particles;
+    (void)origin;
+    if (K < 0 || K > 8) return;
+    const size_t G = (size_t)gdim.x * gdim.y * gdim.z;
+    if (G == 0) return;
+    std::vector<double> tab = dir_table64(K, cell_h);
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, tab.size() * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        pppm_set_last_error("BiotSavartComputeFarField: device allocation failed", __FILE__, __LINE__);
+        return;
+    }
+    (void)hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_pm_far_step1, dim3(blocks(G)), dim3(256), 0, 0, vrho_x, vrho_y, vrho_z, g_u, g_v, g_w, u, v,
+                       w, d, K, (int)gdim.x, (int)gdim.y, (int)gdim.z, cell_h);
+    const double s = cell_h * cell_h;
+    hipLaunchKernelGGL(k_pm_diag_psi, dim3(blocks(G)), dim3(256), 0, 0, vrho_x, psi_x, G, s);
+    hipLaunchKernelGGL(k_pm_diag_psi, dim3(blocks(G)), dim3(256), 0, 0, vrho_y, psi_y, G, s);
+    hipLaunchKernelGGL(k_pm_diag_psi, dim3(blocks(G)), dim3(256), 0, 0, vrho_z, psi_z, G, s);
+    getCurl(psi_x, psi_y, psi_z, g_u, g_v, g_w, gdim.x, gdim.y, gdim.z, cell_h);
+    hipLaunchKernelGGL(k_pm_sub, dim3(blocks(G)), dim3(256), 0, 0, u, g_u, G);
+    hipLaunchKernelGGL(k_pm_sub, dim3(blocks(G)), dim3(256), 0, 0, v, g_v, G);
+    hipLaunchKernelGGL(k_pm_sub, dim3(blocks(G)), dim3(256), 0, 0, w, g_w, G);
+    (void)hipFree(d);  // synchronises the device, as the reference's cudaFree does
+}
+
+extern "C" void BiotSavartInterpolateFarField(float4* pos, double* far_u, double* far_v, double* far_w, double* p_u,
+                                              double* p_v, double* p_w, double cell_h, uint32_t dimx, uint32_t dimy,
+                                              uint32_t dimz, uint32_t num_particles, double4 origin) {
+    if (num_particles == 0) return;
+    hipLaunchKernelGGL(k_pm_interp_far, dim3(blocks(num_particles)), dim3(256), 0, 0, pos, far_u, far_v, far_w, p_u,
+                       p_v, p_w, 1.0 / cell_h, (int)dimx, (int)dimy, (int)dimz, num_particles, origin.x, origin.y,
+                       origin.z);
+}
