@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="served,mixed,fp32")
     ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run an identity gather of 16M float4 rows (4 B index + 16 B read, 16 B written "
+                         "per row) three times, the known-size kernel that HBM counters are scaled by")
     return ap.parse_args()
 
 
@@ -288,6 +291,15 @@ def main():
         }
     if "fp32" in res:
         out["vs_plain_fp32"] = value / out["fp32_path"]["value"] if head != "fp32" else 1.0
+    if a.copy_calibration:
+        rows = 16 << 20
+        idx = N.DeviceBuffer.from_array(np.arange(rows, dtype=np.uint32))
+        src, dst = N.DeviceBuffer(16 * rows), N.DeviceBuffer(16 * rows)
+        for _ in range(3):
+            N.check(N.lib().pppm_gather(idx.ptr, src.ptr, dst.ptr, N.C.c_uint32(4), N.C.c_int32(N.F32),
+                                        N.C.c_uint32(rows), None), "gather")
+        N.synchronize()
+        log(f"[bench] copy calibration: identity gather of {rows} float4 rows x3")
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -180,6 +180,7 @@ typedef struct pppm_request_result {
     double latency_ms;       /* submit to completion                                 */
     double queue_ms;         /* submit to start                                      */
     pppm_stats stats;        /* solver statistics of the request                     */
+    char message[160];       /* pppm_last_error() of the failed call, else empty      */
 } pppm_request_result;
 typedef struct pppm_server_stats {
     uint64_t submitted, completed, failed, particles;

@@ -34,12 +34,20 @@ extern "C" const char* pppm_last_error(void) { return g_last_error.c_str(); }
         if (_s != PPPM_OK) return _s;        \
     } while (0)
 
+namespace pppm {
+std::mutex& runtime_mutex() {
+    static std::mutex m;
+    return m;
+}
+}  // namespace pppm
+
 namespace {
 
 template <typename T>
 pppm_status dev_alloc(T** p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
+    std::lock_guard<std::mutex> lk(pppm::runtime_mutex());
     if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
@@ -49,7 +57,10 @@ pppm_status dev_alloc(T** p, size_t count) {
 }
 template <typename T>
 void dev_free(T*& p) {
-    if (p) (void)hipFree((void*)p);
+    if (p) {
+        std::lock_guard<std::mutex> lk(pppm::runtime_mutex());
+        (void)hipFree((void*)p);
+    }
     p = nullptr;
 }
 
@@ -209,6 +220,7 @@ struct pppm_ctx {
     // Records body() on the context stream into an executable graph.
     template <typename F>
     pppm_status capture(hipGraphExec_t* exec, F body) {
+        std::lock_guard<std::mutex> lk(runtime_mutex());
         PPPM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         pppm_status s = body();
         hipGraph_t gr = nullptr;
@@ -232,15 +244,21 @@ struct pppm_ctx {
         pg = make_geom(p);
         g = p.g;
         G = (size_t)g * g * g;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
-            FAIL(PPPM_ERR_HIP, "stream creation failed");
+        {
+            std::lock_guard<std::mutex> lk(runtime_mutex());
+            if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+                FAIL(PPPM_ERR_HIP, "stream creation failed");
+        }
         own_stream = true;
         TRY(dev_alloc(&cstart, G));
         TRY(dev_alloc(&cend, G));
         TRY(dev_alloc(&coff, G + 1));
         TRY(dev_alloc(&scan_scr, scan_scratch_words(G) + 16));
         TRY(dev_alloc(&d_strays, 4));
-        if (hipHostMalloc((void**)&h_strays, 16) != hipSuccess) FAIL(PPPM_ERR_NOMEM, "pinned alloc failed");
+        {
+            std::lock_guard<std::mutex> lk(runtime_mutex());
+            if (hipHostMalloc((void**)&h_strays, 16) != hipSuccess) FAIL(PPPM_ERR_NOMEM, "pinned alloc failed");
+        }
         TRY(dev_alloc(&wg, 3 * G));
         TRY(dev_alloc(&far, 3 * G));
         if (p.precision == PPPM_PREC_MIXED) {
@@ -277,7 +295,10 @@ struct pppm_ctx {
         dev_free(coff);
         dev_free(scan_scr);
         dev_free(d_strays);
-        if (h_strays) (void)hipHostFree(h_strays);
+        if (h_strays) {
+            std::lock_guard<std::mutex> lk(runtime_mutex());
+            (void)hipHostFree(h_strays);
+        }
         h_strays = nullptr;
         dev_free(epos);
         dev_free(eoth);
@@ -291,7 +312,10 @@ struct pppm_ctx {
         dev_free(r32);
         dev_free(e32);
         mg.release();
-        if (own_stream && st) (void)hipStreamDestroy(st);
+        if (own_stream && st) {
+            std::lock_guard<std::mutex> lk(runtime_mutex());
+            (void)hipStreamDestroy(st);
+        }
         st = nullptr;
     }
 
@@ -762,6 +786,7 @@ extern "C" pppm_status pppm_device_count(int32_t* n) {
 }
 extern "C" pppm_status pppm_malloc(void** ptr, size_t bytes) {
     if (!ptr) FAIL(PPPM_ERR_INVALID, "null ptr");
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
         *ptr = nullptr;
         (void)hipGetLastError();
@@ -770,22 +795,27 @@ extern "C" pppm_status pppm_malloc(void** ptr, size_t bytes) {
     return PPPM_OK;
 }
 extern "C" pppm_status pppm_free(void* ptr) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (ptr) PPPM_HIP_TRY(hipFree(ptr));
     return PPPM_OK;
 }
 extern "C" pppm_status pppm_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (bytes) PPPM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return PPPM_OK;
 }
 extern "C" pppm_status pppm_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (bytes) PPPM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return PPPM_OK;
 }
 extern "C" pppm_status pppm_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (bytes) PPPM_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return PPPM_OK;
 }
 extern "C" pppm_status pppm_memset(void* dst, int value, size_t bytes) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (bytes) PPPM_HIP_TRY(hipMemset(dst, value, bytes));
     return PPPM_OK;
 }

@@ -733,6 +733,7 @@ size_t MGHier<T>::bytes_for(int nx, int ny, int nz, int ncomp) {
 template <typename T>
 pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
     release();
+    std::lock_guard<std::mutex> lk(runtime_mutex());
     if (nx < 1 || ny < 1 || nz < 1 || ncomp < 1 || ncomp > 3) return PPPM_ERR_INVALID;
     ncomp_ = ncomp;
     // MultiGrid.h:115-127 (m_ComputeLevels), 93-98
@@ -760,7 +761,7 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
         for (int l = 0; l < L_; l++) {
             size_t n = cells(l) * ncomp_;
             if (hipMalloc(&t_[l], n * sizeof(T)) != hipSuccess) {
-                release();
+                release_locked();
                 return PPPM_ERR_NOMEM;
             }
             bytes_ += n * sizeof(T);
@@ -769,7 +770,7 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
     for (int l = 1; l < L_; l++) {
         size_t n = cells(l) * ncomp_;
         if (hipMalloc(&x_[l], n * sizeof(T)) != hipSuccess || hipMalloc(&b_[l], n * sizeof(T)) != hipSuccess) {
-            release();
+            release_locked();
             return PPPM_ERR_NOMEM;
         }
         bytes_ += 2 * n * sizeof(T);
@@ -781,7 +782,7 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
         hipMalloc(&tab_ref_, tr.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&part_, (size_t)kNormBlocks * 3 * 2 * sizeof(double) + 6 * sizeof(double)) != hipSuccess ||
         hipHostMalloc(&hnorm_, 8 * sizeof(double)) != hipSuccess) {
-        release();
+        release_locked();
         return PPPM_ERR_NOMEM;
     }
     bytes_ += (tc.size() + tr.size()) * sizeof(double) + (size_t)kNormBlocks * 3 * 2 * sizeof(double) +
@@ -793,6 +794,12 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
 
 template <typename T>
 void MGHier<T>::release() {
+    std::lock_guard<std::mutex> lk(runtime_mutex());
+    release_locked();
+}
+
+template <typename T>
+void MGHier<T>::release_locked() {
     for (T* p : x_)
         if (p) (void)hipFree(p);
     for (T* p : b_)
@@ -1014,9 +1021,9 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
         return a;
     };
     // Blocked passes pay a fixed latency per plane that small levels cannot
-    // amortise: levels with fewer than 2M cells run the one-launch-per-colour
-    // kernels in place instead.
-    auto blocked = [&](int l) { return m.cells(l) >= ((size_t)1 << 21); };
+    // amortise: levels with fewer than 4M cells (128^3 and below) run the
+    // one-launch-per-colour kernels in place instead.
+    auto blocked = [&](int l) { return m.cells(l) >= ((size_t)1 << 22); };
     // down: nu1 passes per level, x from X (or zero below the finest level)
     // ping-ponging with the level's temporary; the last pass also restricts
     std::vector<float*> cur(L, nullptr);
@@ -1076,7 +1083,7 @@ pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T
                               hipStream_t st) {
     bool even = true;  // the blocked prolongation takes parents as i / 2
     for (int l = 0; l + 1 < L_; l++)
-        if (cells(l) >= ((size_t)1 << 21)) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
+        if (cells(l) >= ((size_t)1 << 22)) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
     if (std::is_same<T, float>::value && consistent && tb_ && (nu1 == 1 || nu1 == 2) && (nu2 == 1 || nu2 == 2) &&
         L_ > 1 && even) {
         std::vector<T*> X = x_, B = b_;

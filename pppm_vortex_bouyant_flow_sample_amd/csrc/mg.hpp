@@ -71,6 +71,7 @@ class MGHier {
     bool blocked_smoother() const { return tb_; }
 
    private:
+    void release_locked();  // release() with runtime_mutex() already held
     int L_ = 0, ncomp_ = 1;
     int d_[16][3];
     std::vector<T*> x_, b_;       // levels >= 1

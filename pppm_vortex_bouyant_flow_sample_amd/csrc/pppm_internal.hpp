@@ -1,9 +1,17 @@
 // Internal declarations shared by the library's translation units.
 #pragma once
+#include <mutex>
+
 #include "pppm_common.hpp"
 #include "../../include/pppm.h"
 
 namespace pppm {
+
+// Serialises HIP graph capture against device allocation, release and
+// synchronous copies anywhere in the process: the runtime may invalidate a
+// capture (or refuse the call) when they overlap, and the request server runs
+// captures and context creation or eviction on different threads.
+std::mutex& runtime_mutex();
 
 // ---- scan ----------------------------------------------------------------
 size_t scan_scratch_words(uint64_t n);

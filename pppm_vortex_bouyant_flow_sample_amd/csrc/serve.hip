@@ -24,6 +24,7 @@
 #include <cmath>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -55,6 +56,7 @@ struct Req {
     size_t need = 0;  // bytes of a fresh context for this request
     double t_submit = 0, t_start = 0, t_end = 0;
     pppm_stats stats{};
+    char message[160] = {0};
 };
 
 struct Slot {
@@ -189,6 +191,7 @@ struct pppm_server {
             lk.unlock();
             bool created = false;
             pppm_status st = run(s, q, &created);
+            if (st != PPPM_OK) std::snprintf(q.message, sizeof(q.message), "%s", pppm_last_error());
             size_t have = 0;
             if (s->ctx) (void)pppm_get_device_bytes(s->ctx, &have);
             lk.lock();
@@ -337,6 +340,7 @@ extern "C" pppm_status pppm_server_wait(pppm_server* s, uint64_t ticket, pppm_re
         res->latency_ms = 1e3 * (q.t_end - q.t_submit);
         res->queue_ms = q.t_start > 0 ? 1e3 * (q.t_start - q.t_submit) : 0.0;
         res->stats = q.stats;
+        std::memcpy(res->message, q.message, sizeof(res->message));
     }
     s->reqs.erase(it);
     return PPPM_OK;

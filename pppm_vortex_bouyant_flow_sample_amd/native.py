@@ -282,7 +282,7 @@ class Request(C.Structure):
 
 class RequestResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("pad", C.c_int32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
-                ("stats", Stats)]
+                ("stats", Stats), ("message", C.c_char * 160)]
 
 
 class ServerStats(C.Structure):

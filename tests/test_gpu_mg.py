@@ -78,11 +78,11 @@ def test_blocked_smoother_equals_halfsweep_kernel(native, dims, iters):
     assert np.max(np.abs(out[0] - out[1])) <= 1e-5 * np.max(np.abs(out[0]))
 
 
-@pytest.mark.parametrize("dims,nu", [((128, 128, 128), (2, 2)), ((256, 128, 64), (1, 2)), ((160, 128, 112), (2, 1)),
+@pytest.mark.parametrize("dims,nu", [((256, 128, 128), (2, 2)), ((256, 256, 64), (1, 2)), ((192, 160, 144), (2, 1)),
                                      ((32, 32, 32), (2, 2))])
 def test_fused_vcycle_equals_unfused(native, oracle, dims, nu):
     """Fused V-cycles (blocked smoothing with residual/restriction and
-    prolongation folded in on levels of at least 2M cells, one launch per
+    prolongation folded in on levels of at least 4M cells, one launch per
     colour below) equal the launch-per-step V-cycle to rounding."""
     xs, b = _manufactured(oracle, dims, 9)
     b = b.astype(np.float32)

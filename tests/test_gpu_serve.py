@@ -64,8 +64,8 @@ def _serve(native, items, slots, budget, graphs=True):
     res = [srv.wait(t) for t in tickets]
     st = srv.stats()
     srv.close()
-    for r in res:
-        assert r.status == 0, native.STATUS.get(r.status)
+    for i, r in enumerate(res):
+        assert r.status == 0, f"request {i}: {native.STATUS.get(r.status)}: {r.message.decode()}"
     return [b.result() for b in bufs], res, st
 
 
