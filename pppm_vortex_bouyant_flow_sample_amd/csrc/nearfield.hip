@@ -61,7 +61,10 @@ __device__ inline int near_hi_cell(float v, double rb, double invh, int gi) {
 // union of the points' boxes; each row segment (consecutive x cells are
 // consecutive sorted vortices) is staged 64 vortices at a time into LDS with
 // coalesced loads, and each lane tests its slice of the chunk against its
-// point, evaluating the kernel where the box test passes.  The M2P corners of
+// point, evaluating the kernel where the box test passes.  The row ranges of
+// the box are read in one vector load (a row per lane) and the chunks run as
+// one stream across rows, the next chunk's loads in flight while the current
+// one is tested.  The M2P corners of
 // a point are shared among its lanes; partial sums are reduced over the
 // slices at the end (fixed order, deterministic).  All loop bounds are
 // wave-uniform.
@@ -131,46 +134,87 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
         const uint32_t nsrc = off[G];
         float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f);
         if (SEGMENT) bpos = eoth[tv ? t : t0];
-        if (xlo <= xhi) {
-            for (int kk = zlo; kk <= zhi; kk++) {
-                for (int jj = ylo; jj <= yhi; jj++) {
-                    const uint32_t rowc = ((uint32_t)kk * g + (uint32_t)jj) * g;
-                    const uint32_t p0 = __builtin_amdgcn_readfirstlane(off[rowc + xlo]);
-                    const uint32_t p1 = __builtin_amdgcn_readfirstlane(min(off[rowc + xhi + 1], nsrc));
-                    for (uint32_t base = p0; base < p1; base += 64) {
-                        const uint32_t cnt = min(64u, p1 - base);
-                        if ((uint32_t)lane < cnt) {
-                            cpos[w][lane] = spos[base + lane];
-                            comg[w][lane] = somg[base + lane];
-                        }
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                        for (uint32_t q = sj; q < cnt; q += kNearSJ) {
-                            const float4 s = cpos[w][q];
-                            const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
-                            bool acc = tv && fabsf(dx) <= rbox && fabsf(dy) <= rbox && fabsf(dz) <= rbox;
-                            const float r2 = dx * dx + dy * dy + dz * dz;
-                            acc = acc && r2 >= 1e-24f;
-                            if (SEGMENT) {
-                                const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
-                                acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
-                            }
-                            if (acc) {
-                                const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
-                                                                                : coeff_expcubic(r2, nc);
-                                const float4 o = comg[w][q];
-                                vx += (o.y * dz - o.z * dy) * cf;
-                                vy += (o.z * dx - o.x * dz) * cf;
-                                vz += (o.x * dy - o.y * dx) * cf;
-                            }
-                        }
-                        // the next chunk overwrites the staging slots
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int ny_ = yhi - ylo + 1;
+        const int nrows = (xlo <= xhi && ylo <= yhi && zlo <= zhi) ? (zhi - zlo + 1) * ny_ : 0;
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+            // lane l holds the vortex range [lp0, lp1) of row r0 + l (z outer,
+            // y inner), loaded in one pass instead of row after row
+            uint32_t lp0 = 0, lp1 = 0;
+            if (r0 + lane < nrows) {
+                const int r = r0 + lane;
+                const uint32_t kk = (uint32_t)(zlo + r / ny_), jj = (uint32_t)(ylo + r % ny_);
+                const uint32_t rowc = (kk * g + jj) * g;
+                lp0 = off[rowc + xlo];
+                lp1 = min(off[rowc + xhi + 1], nsrc);
+            }
+            const int rn = min(64, nrows - r0);
+            // chunk cursor (row cr, vortices [cb, min(cb + 64, ce))), wave-uniform
+            auto advance = [&](int& r, uint32_t& b, uint32_t& e) {
+                b += 64;
+                while (b >= e && r < rn) {
+                    r++;
+                    if (r < rn) {
+                        b = __builtin_amdgcn_readlane(lp0, r);
+                        e = __builtin_amdgcn_readlane(lp1, r);
                     }
                 }
+            };
+            int cr = 0;
+            uint32_t cb = __builtin_amdgcn_readlane(lp0, 0), ce = __builtin_amdgcn_readlane(lp1, 0);
+            if (cb >= ce) {
+                cb -= 64;  // wraps; advance adds it back and skips the empty rows
+                advance(cr, cb, ce);
+            }
+            // the next chunk is loaded into registers while this one is tested
+            float4 pp = make_float4(0.f, 0.f, 0.f, 0.f), po = pp;
+            if (cr < rn && cb + lane < ce) {
+                pp = spos[cb + lane];
+                po = somg[cb + lane];
+            }
+            while (cr < rn) {
+                const uint32_t cnt = min(64u, ce - cb);
+                cpos[w][lane] = pp;
+                comg[w][lane] = po;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                int nr = cr;
+                uint32_t nb = cb, ne = ce;
+                advance(nr, nb, ne);
+                if (nr < rn && nb + lane < ne) {
+                    pp = spos[nb + lane];
+                    po = somg[nb + lane];
+                }
+#pragma unroll
+                for (int qq = 0; qq < kNearTI; qq++) {
+                    const uint32_t q = sj + qq * kNearSJ;
+                    if (q < cnt) {
+                        const float4 s = cpos[w][q];
+                        const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
+                        bool acc = tv && fabsf(dx) <= rbox && fabsf(dy) <= rbox && fabsf(dz) <= rbox;
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        acc = acc && r2 >= 1e-24f;
+                        if (SEGMENT) {
+                            const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
+                            acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
+                        }
+                        if (acc) {
+                            const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
+                                                                            : coeff_expcubic(r2, nc);
+                            const float4 o = comg[w][q];
+                            vx += (o.y * dz - o.z * dy) * cf;
+                            vy += (o.z * dx - o.x * dz) * cf;
+                            vz += (o.x * dy - o.y * dx) * cf;
+                        }
+                    }
+                }
+                // the next chunk overwrites the staging slots
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                cr = nr;
+                cb = nb;
+                ce = ne;
             }
         }
     }
