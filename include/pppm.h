@@ -238,6 +238,12 @@ pppm_status pppm_scatter(const uint32_t *idx, const void *in, void *out, uint32_
 typedef struct pppm_mg pppm_mg;
 /* dtype PPPM_F32 or PPPM_F64; nx,ny,nz arbitrary >= 1 */
 pppm_status pppm_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, pppm_mg **out);
+/* Batched form: x and b hold ncomp (1..3) grids one after another
+ * (component-major), solved together with one launch per pass over all of
+ * them, as the three vector-potential components of BiotSavartSolverImp.cpp:
+ * 603-620 (m_SolvePoisson, there three sequential solves).  pppm_mg_solve
+ * then reports the worst component's relative residual. */
+pppm_status pppm_mg_create_batched(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, int32_t ncomp, pppm_mg **out);
 void pppm_mg_destroy(pppm_mg *mg);
 pppm_status pppm_mg_levels(const pppm_mg *mg, int32_t *levels, int32_t dims_out[][3], int32_t max_levels);
 /* Solve sum_nb(x) - 6 x = b (zero ghosts) in place on x.  schedule
@@ -251,13 +257,20 @@ pppm_status pppm_mg_solve(pppm_mg *mg, void *x, const void *b, int32_t schedule,
                           double *relres_out, void *hip_stream);
 
 /* iters red-black SOR iterations (odd colour first, MultiGrid_kernel.cu:722-820)
- * on the finest level, in place on x; blocked = 1 uses the temporally blocked
- * LDS smoother (PPPM_F32 only), 0 one launch per half-sweep. */
+ * on the finest level, in place on x.  blocked (PPPM_F32 only): 0 one launch
+ * per half-sweep; 1 temporally blocked LDS passes; 2 colour-split blocked
+ * passes (four-plane ring); 3 the same with a six-plane ring.  Kinds 2 and 3
+ * take kind 1's kernel when nx is odd. */
 pppm_status pppm_mg_smooth(pppm_mg *mg, void *x, const void *b, int32_t iters, double omega, int32_t blocked,
                            void *hip_stream);
-/* blocked = 1 (default): PPPM_F32 V-cycles use the fused temporally blocked
- * passes; 0: one launch per half-sweep, residual, restriction, prolongation. */
+/* PPPM_F32 V-cycles on levels at or above the blocked threshold (below):
+ * blocked = 1..3 (the kinds of pppm_mg_smooth; default 2) use fused blocked
+ * passes, 0 one launch per half-sweep, residual, restriction and
+ * prolongation. */
 pppm_status pppm_mg_set_blocked(pppm_mg *mg, int32_t blocked);
+/* Levels with at least this many cells per component take the blocked
+ * passes in PPPM_F32 V-cycles (default 256K: 64^3 and larger). */
+pppm_status pppm_mg_set_blocked_min_cells(pppm_mg *mg, uint64_t cells);
 
 /* ---- device memory (MemoryOperation.cuh:13-29 semantics) -------------------- */
 pppm_status pppm_device_count(int32_t *n);

@@ -346,6 +346,337 @@ __global__ void __launch_bounds__(256) k_rb2(TbArgs a) {
     }
 }
 
+// Colour-split blocked smoother: the pass of k_rb2 (one red-black iteration
+// per pass, z planes streamed through LDS rings, halos recomputed, out of
+// place) with each LDS plane split by colour.  Row ly of colour c holds at
+// index m the colour-c cell of the cell pair (2m, 2m + 1) of that row.  Every
+// neighbour of a colour-c cell is of colour 1 - c: at index m in the rows
+// above and below and in the planes before and after, and at m + s - 1 and
+// m + s in its own row (s = 1 if the cell is the pair's second), so the reads
+// of a half-sweep are contiguous across lanes.  A thread owns whole pairs: it
+// loads them with 8-byte loads, keeps their right-hand side in registers and
+// writes its own cells out.  The z loop is unrolled over the ring (NR planes,
+// even, the loop aligned to multiples of NR), so ring slots and plane
+// parities are compile-time and every LDS address is a per-slot register plus
+// an immediate.  Planes outside the grid in z read as zero: the ring slot
+// before the first plane is still zero while the first plane is swept, and a
+// zero plane is stored after the last one.  Cells outside the grid in x and y
+// are loaded as zero and keep zero (their relaxation factor is 0).  Rows
+// beyond the loaded region read zero rows, and an x neighbour beyond a row's
+// end reads the adjacent row's cell (rows are not padded, so a wave's reads
+// cover 64 consecutive words); the values these produce at the loaded
+// region's edge reach at most one cell further inward per half-sweep, which
+// the halo absorbs.  Requires nx even: pairs never cross the x boundary, and
+// the x halo is even for the same reason.
+constexpr int kRcTX = 64;
+constexpr int kRcTY = 16;
+
+template <bool RESTRICT, bool PROLONG, bool XZERO, bool BETA, int NR>
+__global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
+    constexpr int HXC = RESTRICT ? 4 : 2;        // x halo in cells (even)
+    constexpr int HY = RESTRICT ? 3 : 2;         // y and z halo
+    constexpr int LX = kRcTX + 2 * HXC, LY = kRcTY + 2 * HY;
+    constexpr int HX = LX / 2, HXP = HX;         // pairs per row (no padding: a wave's reads stay in one bank set)
+    constexpr int CP = (LY + 2) * HXP;           // one colour of one plane: rows -1 .. LY
+    constexpr int PS = 2 * CP;                   // one plane, both colours
+    constexpr int NSLOT = HX * LY;
+    constexpr int NP = (NSLOT + 255) / 256;      // pair slots per thread
+    constexpr int CLX = PROLONG ? LX / 2 + 3 : 1, CLY = PROLONG ? LY / 2 + 3 : 1;
+    constexpr int NC = (CLX * CLY + 255) / 256;
+    constexpr int RW = kRcTX / 2;                // restriction: coarse cells per tile row
+    // the next step's store reuses a ring slot this step reads, or the
+    // residual reads what the second half-sweep wrote: barrier after it
+    constexpr bool kBarAfter = RESTRICT || NR < 5;
+    static_assert(NR % 2 == 0 && NR >= (RESTRICT ? 6 : 4), "ring too short");
+    static_assert(256 * (NP - 1) <= NSLOT, "only the last slot may be partial");
+    __shared__ float xs[NR * PS];
+    __shared__ float rs[RESTRICT ? kRcTY * RW : 1];
+    __shared__ float cw[PROLONG ? 3 : 1][CLY][CLX];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const uint32_t tile = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                    gridDim.x * gridDim.y * gridDim.z);
+    const int bx = (int)(tile % gridDim.x), by = (int)((tile / gridDim.x) % gridDim.y);
+    const int bz = (int)(tile / (gridDim.x * gridDim.y));
+    const int gx0 = bx * kRcTX - HXC, gy0 = by * kRcTY - HY;
+    const int nzc = (nz + a.zc - 1) / a.zc;
+    const int comp = bz / nzc, zi = bz - comp * nzc;
+    const int z0 = zi * a.zc, z1 = min(z0 + a.zc, nz);
+    const float* xsrc = a.xin + (size_t)comp * a.G;
+    float* xdst = a.x + (size_t)comp * a.G;
+    const float* bcp = a.b + (size_t)comp * a.G;
+    const float* xcoarse = PROLONG ? a.xc + (size_t)comp * a.Gc : nullptr;
+    const int cx0 = (gx0 >> 1) - 1, cy0 = (gy0 >> 1) - 1;  // coarse window origin
+    const int klo = max(z0 - HY, 0), khi = min(z1 + HY, nz);
+    const int kend = khi + 2 + (RESTRICT ? 1 : 0);
+    const size_t plane = (size_t)nx * ny;
+    const float omega = a.omega, beta = a.beta, bscale = a.bscale;
+    const int tid = threadIdx.x;
+
+    for (int q = tid; q < NR * PS; q += 256) xs[q] = 0.f;
+
+    // Pair slots of this thread: q = tid + 256 u -> row ly, pair m.
+    //   e:      index of the pair in a colour plane
+    //   ea[v]:  the pair's first cell in its colour plane when the plane
+    //           parity is v (that cell's colour is (gy0 + ly + v) & 1)
+    //   exn[w]: first x neighbour of the pair's colour-c cell in the other
+    //           colour's plane, w = c ^ plane parity
+    //   goff:   the pair's offset in a grid plane (0 off the grid)
+    int e[NP], ea[NP][2], exn[NP][2], rsi[NP];
+    uint32_t gob[NP];  // byte offset of the pair in a grid plane (0 off the grid)
+    float wu[NP];
+    bool sv[NP], gin[NP], io[NP], swp[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+        const int q = tid + 256 * u;
+        const bool valid = q < NSLOT;
+        const int ly = valid ? q / HX : 0, m = valid ? q - ly * HX : 0;
+        const int i0 = gx0 + 2 * m, j = gy0 + ly;
+        const bool in = valid && i0 >= 0 && i0 < nx && j >= 0 && j < ny;
+        const int sp = j & 1;
+        e[u] = (ly + 1) * HXP + m;
+        ea[u][0] = e[u] + (sp ? CP : 0);
+        ea[u][1] = e[u] + (sp ? 0 : CP);
+        exn[u][0] = e[u] + sp - 1;
+        exn[u][1] = e[u] - sp;
+        gob[u] = in ? 4u * (uint32_t)(j * nx + i0) : 0u;
+        wu[u] = in ? omega : 0.f;
+        sv[u] = valid;
+        gin[u] = in;
+        swp[u] = sp != 0;
+        io[u] = in && ly >= HY && ly < HY + kRcTY && m >= HXC / 2 && m < HXC / 2 + kRcTX / 2;
+        rsi[u] = (ly - HY) * RW + (m - HXC / 2);
+    }
+
+    float2 px[NP], pb[NP], bq[NR][NP];
+    float pc[NC];
+    float rsum[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+        px[u] = pb[u] = make_float2(0.f, 0.f);
+        rsum[u] = 0.f;
+#pragma unroll
+        for (int r = 0; r < NR; r++) bq[r][u] = make_float2(0.f, 0.f);
+    }
+    // plane k into px, pb (loaded during the step before its store); k >=
+    // khi gives the zero plane (the plane index goes through readfirstlane so
+    // the compiler keeps one scalar base per plane instead of per-slot 64-bit
+    // pointers)
+    auto fetch = [&](int k) {
+        const size_t ko = (size_t)__builtin_amdgcn_readfirstlane(k) * plane;
+        const char* xp_ = reinterpret_cast<const char*>(xsrc + ko);
+        const char* bp_ = reinterpret_cast<const char*>(bcp + ko);
+        const bool real = k < khi;
+#pragma unroll
+        for (int u = 0; u < NP; u++) {
+            float2 xv = make_float2(0.f, 0.f), bv = make_float2(0.f, 0.f);
+            if (real && gin[u]) {
+                if (!XZERO) xv = *reinterpret_cast<const float2*>(xp_ + gob[u]);
+                bv = *reinterpret_cast<const float2*>(bp_ + gob[u]);
+            }
+            px[u] = xv;
+            pb[u] = bv;
+        }
+    };
+    // Coarse window for the prolongation, ghost-extended: a coarse cell
+    // outside the grid holds betac times its nearest in-grid cell per axis
+    // it lies out on, so the trilinear weights need no boundary cases (the
+    // consistent transfer of k_prolong); plane c in {-1 .. cz} lives in ring
+    // slot (c + 3) % 3.
+    auto fetch_coarse = [&](int c) {
+        const int cc = min(max(c, 0), a.cz - 1);
+        const float fz = cc != c ? a.betac : 1.f;
+#pragma unroll
+        for (int u = 0; u < NC; u++) {
+            const int q = tid + 256 * u;
+            const int cyl = q / CLX, cxl = q - cyl * CLX;
+            const int ci = cx0 + cxl, cj = cy0 + cyl;
+            const int cic = min(max(ci, 0), a.cx - 1), cjc = min(max(cj, 0), a.cy - 1);
+            float v = 0.f;
+            if (q < CLX * CLY)
+                v = xcoarse[((size_t)cc * a.cy + cjc) * a.cx + cic] * ((cic != ci ? a.betac : 1.f) *
+                                                                         (cjc != cj ? a.betac : 1.f) * fz);
+            pc[u] = v;
+        }
+    };
+    auto store_coarse = [&](int c) {
+#pragma unroll
+        for (int u = 0; u < NC; u++) {
+            const int q = tid + 256 * u;
+            if (q < CLX * CLY) cw[(c + 3) % 3][q / CLX][q % CLX] = pc[u];
+        }
+    };
+    if (PROLONG) {
+        for (int c = (klo >> 1) - 1; c <= (klo >> 1) + 1; c++) {
+            fetch_coarse(c);
+            store_coarse(c);
+        }
+    }
+    fetch(klo);
+    __syncthreads();  // zeroed rings before the first stores
+
+    for (int k0 = klo - klo % NR; k0 < kend; k0 += NR) {
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            const int k = k0 + j;  // k % NR == j, k & 1 == j & 1
+            if (k < klo || k >= kend) continue;
+            if (PROLONG) {
+                if (k < khi && k > klo && (j & 1) == 0) store_coarse((k >> 1) + 1);  // fetched during plane k - 1
+                __syncthreads();
+            }
+            // plane k into ring slot j (the zero plane at k == khi)
+            if (k <= khi) {
+                float* P = xs + j * PS;
+#pragma unroll
+                for (int u = 0; u < NP; u++) {
+                    if (u == NP - 1 && !sv[u]) continue;
+                    float2 xv = px[u];
+                    if (PROLONG && k < khi && gin[u]) {
+                        // trilinear prolongation of the pair: parent column m + 1
+                        // of the window, neighbours m (first cell) and m + 2
+                        // (second); parent row (ly >> 1) + 1, neighbour one
+                        // up (ly odd) or down; parent plane k >> 1, neighbour
+                        // one up (k odd) or down
+                        const int m = e[u] % HXP, ly = e[u] / HXP - 1;
+                        const int rp = (ly >> 1) + 1, rn = rp + ((ly & 1) ? 1 : -1);
+                        const int pk = k >> 1, nk = pk + ((j & 1) ? 1 : -1);
+                        const float* cp0 = &cw[(pk + 3) % 3][0][0];
+                        const float* cn0 = &cw[(nk + 3) % 3][0][0];
+                        float ye[2], yo[2];
+#pragma unroll
+                        for (int z = 0; z < 2; z++) {
+                            const float* cz_ = z ? cn0 : cp0;
+                            const float* a0 = cz_ + rp * CLX + m;
+                            const float* a1 = cz_ + rn * CLX + m;
+                            const float xe0 = 0.75f * a0[1] + 0.25f * a0[0], xo0 = 0.75f * a0[1] + 0.25f * a0[2];
+                            const float xe1 = 0.75f * a1[1] + 0.25f * a1[0], xo1 = 0.75f * a1[1] + 0.25f * a1[2];
+                            ye[z] = 0.75f * xe0 + 0.25f * xe1;
+                            yo[z] = 0.75f * xo0 + 0.25f * xo1;
+                        }
+                        xv.x += 0.75f * ye[0] + 0.25f * ye[1];
+                        xv.y += 0.75f * yo[0] + 0.25f * yo[1];
+                    }
+                    P[ea[u][j & 1]] = xv.x;
+                    P[ea[u][(j & 1) ^ 1]] = xv.y;
+                    // b in colour order: .x colour 0, .y colour 1
+                    const bool sw = swp[u] != ((j & 1) != 0);
+                    bq[j][u] = sw ? make_float2(pb[u].y, pb[u].x) : pb[u];
+                }
+            }
+            __syncthreads();
+            if (k + 1 <= khi) {
+                fetch(k + 1);
+                if (PROLONG && k + 1 < khi && (j & 1) == 1) fetch_coarse(((k + 1) >> 1) + 1);
+            }
+#pragma unroll
+            for (int t = 1; t <= 2; t++) {
+                const int p = k - t;
+                const int plo = klo > 0 ? klo + t : 0;
+                const int phi = khi < nz ? khi - 1 - t : nz - 1;
+                if (p >= plo && p <= phi) {
+                    const int jp = (j - t + 2 * NR) % NR, jm = (jp + NR - 1) % NR, jq = (jp + 1) % NR;
+                    const int c = t & 1;  // odd colour first, as the reference
+                    const int w = c ^ (jp & 1);
+                    const float* PB = xs + jp * PS + (1 - c) * CP;
+                    const float* PM = xs + jm * PS + (1 - c) * CP;
+                    const float* PQ = xs + jq * PS + (1 - c) * CP;
+                    float* PC = xs + jp * PS + c * CP;
+                    // all reads of the half-sweep before its writes: the
+                    // slots' cells are distinct and no update reads another
+                    // slot's cell of the colour being written
+                    float xos[NP], rr[NP];
+#pragma unroll
+                    for (int u = 0; u < NP; u++) {
+                        if (u == NP - 1 && !sv[u]) continue;
+                        const int o = e[u], ox = exn[u][w];
+                        xos[u] = PC[o];
+                        float r = PB[ox] + PB[ox + 1];
+                        r += PB[o - HXP] + PB[o + HXP];
+                        r += PM[o] + PQ[o];
+                        rr[u] = r;
+                    }
+#pragma unroll
+                    for (int u = 0; u < NP; u++) {
+                        if (u == NP - 1 && !sv[u]) continue;
+                        const int o = e[u];
+                        const float xo = xos[u];
+                        float r = fmaf(-bscale, c ? bq[jp][u].y : bq[jp][u].x, rr[u]);
+                        float d = 1.0f / 6.0f;
+                        if (BETA) {
+                            const int m = o % HXP, ly = o / HXP - 1;
+                            const int i = gx0 + 2 * m + ((swp[u] ? 1 : 0) ^ w), jj = gy0 + ly;
+                            const int nout = (i == 0) + (i == nx - 1) + (jj == 0) + (jj == ny - 1) + (p == 0) +
+                                             (p == nz - 1);
+                            d = 1.0f / (6.0f - beta * (float)nout);
+                        }
+                        PC[o] = xo + wu[u] * (r * d - xo);
+                    }
+                }
+                if (t == 1 || kBarAfter) __syncthreads();
+            }
+            // plane k - 2 is final: write this thread's interior pairs
+            {
+                const int pw = k - 2;
+                if (pw >= z0 && pw < z1) {
+                    const int jw = (j + 2 * NR - 2) % NR;
+                    const float* P = xs + jw * PS;
+                    char* dst = reinterpret_cast<char*>(xdst + (size_t)__builtin_amdgcn_readfirstlane(pw) * plane);
+#pragma unroll
+                    for (int u = 0; u < NP; u++)
+                        if (io[u])
+                            *reinterpret_cast<float2*>(dst + gob[u]) =
+                                make_float2(P[ea[u][jw & 1]], P[ea[u][(jw & 1) ^ 1]]);
+                }
+            }
+            if (RESTRICT) {
+                // residual of plane pr = k - 3 (its neighbours are final) at this
+                // thread's interior pairs, summed over the pair and the plane
+                // pair, then over the two rows of a coarse cell through rs
+                const int pr = k - 3;
+                if (pr >= z0 && pr < z1) {
+                    const int jr = (j + 2 * NR - 3) % NR, jm = (jr + NR - 1) % NR, jq = (jr + 1) % NR;
+#pragma unroll
+                    for (int u = 0; u < NP; u++) {
+                        if (!io[u]) continue;
+                        float part = 0.f;
+#pragma unroll
+                        for (int c = 0; c < 2; c++) {
+                            const int w = c ^ (jr & 1);
+                            const float* PB = xs + jr * PS + (1 - c) * CP;
+                            const float* PM = xs + jm * PS + (1 - c) * CP;
+                            const float* PQ = xs + jq * PS + (1 - c) * CP;
+                            const float* PC = xs + jr * PS + c * CP;
+                            const int o = e[u], ox = exn[u][w];
+                            const float xo = PC[o];
+                            float sres = PB[ox] + PB[ox + 1];
+                            sres += PB[o - HXP] + PB[o + HXP];
+                            sres += PM[o] + PQ[o];
+                            sres -= 6.0f * xo;
+                            if (BETA) {
+                                const int m = o % HXP, ly = o / HXP - 1;
+                                const int i = gx0 + 2 * m + ((swp[u] ? 1 : 0) ^ w), jj = gy0 + ly;
+                                const int nout = (i == 0) + (i == nx - 1) + (jj == 0) + (jj == ny - 1) + (pr == 0) +
+                                                 (pr == nz - 1);
+                                sres += beta * (float)nout * xo;
+                            }
+                            part += fmaf(bscale, c ? bq[jr][u].y : bq[jr][u].x, -sres);
+                        }
+                        if ((jr & 1) == 0) rsum[u] = part;
+                        else rs[rsi[u]] = rsum[u] + part;
+                    }
+                    if (jr & 1) {
+                        __syncthreads();
+                        const int I = tid % RW, J = tid / RW;
+                        const int gI = bx * RW + I, gJ = by * (kRcTY / 2) + J, gK = pr / 2;
+                        if (gI < a.cx && gJ < a.cy && gK < a.cz)
+                            a.bn[(size_t)comp * a.Gc + ((size_t)gK * a.cy + gJ) * a.cx + gI] =
+                                (rs[(2 * J) * RW + I] + rs[(2 * J + 1) * RW + I]) * 0.5f;
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <typename T>
 __device__ inline T residual_at(const T* __restrict__ x, const T* __restrict__ b, T bscale, int i, int j, int k,
                                 int nx, int ny, int nz, T beta) {
@@ -730,6 +1061,8 @@ size_t MGHier<T>::bytes_for(int nx, int ny, int nz, int ncomp) {
     return b;
 }
 
+static void rbc_prime();
+
 template <typename T>
 pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
     release();
@@ -757,6 +1090,7 @@ pppm_status MGHier<T>::init(int nx, int ny, int nz, int ncomp) {
     t_.assign(L_, nullptr);
     bytes_ = 0;
     if (std::is_same<T, float>::value) {
+        rbc_prime();
         // out-of-place targets of the blocked smoother, every level
         for (int l = 0; l < L_; l++) {
             size_t n = cells(l) * ncomp_;
@@ -855,27 +1189,107 @@ static int tb_zchunk(int nx, int ny, int nz, int ncomp) {
     return std::max(2, std::min(zc, nz + (nz & 1)));
 }
 
-// One launch of k_rb2 over all components.
+// Compute units of the current device (cached per device).
+static int cu_count() {
+    static int n[16] = {0};
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 16) d = 0;
+    if (n[d] == 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || c < 1) c = 256;
+        n[d] = c;
+    }
+    return n[d];
+}
+
+// Launch one k_rbc instantiation with z chunks sized so the whole grid is
+// resident at once (one round of workgroups: no tail), when the tiles leave
+// room for at least two chunks; otherwise the caller's chunking.
+template <bool R, bool P, bool Z, bool B, int NR>
+static int rbc_per_cu() {
+    static int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&k_rbc<R, P, Z, B, NR>),
+                                                         256, 0) != hipSuccess ||
+            n < 1)
+            n = 1;
+        return n;
+    }();
+    return per_cu;
+}
+
+// The occupancy queries run once, when a hierarchy is set up, not inside a
+// stream capture.
+static void rbc_prime() {
+    (void)cu_count();
+    (void)rbc_per_cu<false, false, false, false, 4>();
+    (void)rbc_per_cu<false, false, false, true, 4>();
+    (void)rbc_per_cu<false, false, true, false, 4>();
+    (void)rbc_per_cu<false, false, true, true, 4>();
+    (void)rbc_per_cu<false, true, false, false, 4>();
+    (void)rbc_per_cu<false, true, false, true, 4>();
+    (void)rbc_per_cu<false, false, false, false, 6>();
+    (void)rbc_per_cu<false, false, false, true, 6>();
+    (void)rbc_per_cu<false, false, true, false, 6>();
+    (void)rbc_per_cu<false, false, true, true, 6>();
+    (void)rbc_per_cu<false, true, false, false, 6>();
+    (void)rbc_per_cu<false, true, false, true, 6>();
+    (void)rbc_per_cu<true, false, false, false, 6>();
+    (void)rbc_per_cu<true, false, false, true, 6>();
+    (void)rbc_per_cu<true, false, true, false, 6>();
+    (void)rbc_per_cu<true, false, true, true, 6>();
+}
+
+template <bool R, bool P, bool Z, bool B, int NR>
+static void launch_rbc(TbArgs a, int ncomp, hipStream_t st) {
+    const int per_cu = rbc_per_cu<R, P, Z, B, NR>();
+    const int tiles = (int)(div_up(a.nx, kRcTX) * div_up(a.ny, kRcTY)) * ncomp;
+    const int chunks = per_cu * cu_count() / tiles;
+    if (chunks >= 2) {
+        int zc = (a.nz + chunks - 1) / chunks;
+        zc += zc & 1;
+        a.zc = std::max(2, zc);
+    }
+    const int nzc = (a.nz + a.zc - 1) / a.zc;
+    dim3 grid(div_up(a.nx, kRcTX), div_up(a.ny, kRcTY), nzc * ncomp), blk(256);
+    hipLaunchKernelGGL((k_rbc<R, P, Z, B, NR>), grid, blk, 0, st, a);
+}
+
+// One blocked pass over all components: kind 1 = k_rb2, 2 = k_rbc with a
+// four-plane ring, 3 = k_rbc with a six-plane ring (the residual pass always
+// uses six); k_rbc needs nx even, k_rb2 takes any size.
 template <bool R, bool P, bool Z>
-static pppm_status launch_rb2(const TbArgs& a, int ncomp, hipStream_t st) {
+static pppm_status launch_rb2(const TbArgs& a, int ncomp, int kind, hipStream_t st) {
+    static_assert(kTbTX == kRcTX && kTbTY == kRcTY, "both blocked kernels use the same tiles");
     const int nzc = (a.nz + a.zc - 1) / a.zc;
     dim3 grid(div_up(a.nx, kTbTX), div_up(a.ny, kTbTY), nzc * ncomp), blk(256);
-    if (a.beta != 0.f)
+    const bool beta = a.beta != 0.f;
+    if (kind >= 2 && (a.nx & 1) == 0) {
+        if (R || kind == 3) {
+            if (beta) launch_rbc<R, P, Z, true, 6>(a, ncomp, st);
+            else launch_rbc<R, P, Z, false, 6>(a, ncomp, st);
+        } else {
+            constexpr int NR4 = R ? 6 : 4;
+            if (beta) launch_rbc<R, P, Z, true, NR4>(a, ncomp, st);
+            else launch_rbc<R, P, Z, false, NR4>(a, ncomp, st);
+        }
+    } else if (beta) {
         hipLaunchKernelGGL((k_rb2<R, P, Z, true>), grid, blk, 0, st, a);
-    else
+    } else {
         hipLaunchKernelGGL((k_rb2<R, P, Z, false>), grid, blk, 0, st, a);
+    }
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }
 
 template <typename T>
-static pppm_status smooth_tb(const T*, T*, const T*, T, int, int, int, int, size_t, T, T, hipStream_t) {
+static pppm_status smooth_tb(const T*, T*, const T*, T, int, int, int, int, size_t, T, T, int, hipStream_t) {
     return PPPM_ERR_INVALID;
 }
 // one red-black iteration, xin -> x
 template <>
 pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float bscale, int nx, int ny, int nz,
-                             int ncomp, size_t G, float omega, float beta, hipStream_t st) {
+                             int ncomp, size_t G, float omega, float beta, int kind, hipStream_t st) {
     TbArgs a{};
     a.xin = xin;
     a.x = x;
@@ -888,7 +1302,7 @@ pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float b
     a.omega = omega;
     a.beta = beta;
     a.zc = tb_zchunk(nx, ny, nz, ncomp);
-    return launch_rb2<false, false, false>(a, ncomp, st);
+    return launch_rb2<false, false, false>(a, ncomp, kind, st);
 }
 
 template <typename T>
@@ -913,7 +1327,7 @@ pppm_status MGHier<T>::smooth(int l, T* x, const T* b, T bscale, int iters, T om
         for (int q = 0; q < iters; q++) {
             const T* src = (q & 1) ? tmp : x;
             T* dst = (q & 1) ? x : tmp;
-            pppm_status s = smooth_tb<T>(src, dst, b, bscale, nx, ny, nz, ncomp_, cells(l), omega, beta, st);
+            pppm_status s = smooth_tb<T>(src, dst, b, bscale, nx, ny, nz, ncomp_, cells(l), omega, beta, tb_, st);
             if (s != PPPM_OK) return s;
         }
         if (iters & 1) return copy(x, tmp, cells(l) * ncomp_, st);
@@ -999,7 +1413,7 @@ static pppm_status vcycle_fused(MGHier<T>&, std::vector<T*>&, std::vector<T*>&, 
 template <>
 pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::vector<float*>& B, float bscale0,
                                 int nu1, int nu2, float omega, hipStream_t st) {
-    const int L = m.levels(), nc = m.ncomp();
+    const int L = m.levels(), nc = m.ncomp(), kind = m.blocked_smoother();
     pppm_status s;
     auto level_args = [&](int l) {
         const int* d = m.dims(l);
@@ -1021,9 +1435,9 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
         return a;
     };
     // Blocked passes pay a fixed latency per plane that small levels cannot
-    // amortise: levels with fewer than 4M cells (128^3 and below) run the
+    // amortise: levels below the hierarchy's threshold run the
     // one-launch-per-colour kernels in place instead.
-    auto blocked = [&](int l) { return m.cells(l) >= ((size_t)1 << 22); };
+    auto blocked = [&](int l) { return m.cells(l) >= m.blocked_min_cells(); };
     // down: nu1 passes per level, x from X (or zero below the finest level)
     // ping-ponging with the level's temporary; the last pass also restricts
     std::vector<float*> cur(L, nullptr);
@@ -1044,8 +1458,11 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
             a.x = dst;
             a.bn = B[l + 1];
             const bool zero = l > 0 && q == 0, last = q == nu1 - 1;
-            if (last) s = zero ? launch_rb2<true, false, true>(a, nc, st) : launch_rb2<true, false, false>(a, nc, st);
-            else s = zero ? launch_rb2<false, false, true>(a, nc, st) : launch_rb2<false, false, false>(a, nc, st);
+            if (last)
+                s = zero ? launch_rb2<true, false, true>(a, nc, kind, st) : launch_rb2<true, false, false>(a, nc, kind, st);
+            else
+                s = zero ? launch_rb2<false, false, true>(a, nc, kind, st)
+                         : launch_rb2<false, false, false>(a, nc, kind, st);
             if (s != PPPM_OK) return s;
             std::swap(src, dst);
         }
@@ -1069,7 +1486,8 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
             a.x = dst;
             a.xc = X[l + 1];
             a.betac = (float)mg_beta(l + 1);
-            s = q == 0 ? launch_rb2<false, true, false>(a, nc, st) : launch_rb2<false, false, false>(a, nc, st);
+            s = q == 0 ? launch_rb2<false, true, false>(a, nc, kind, st)
+                       : launch_rb2<false, false, false>(a, nc, kind, st);
             if (s != PPPM_OK) return s;
             std::swap(src, dst);
         }
@@ -1083,7 +1501,7 @@ pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T
                               hipStream_t st) {
     bool even = true;  // the blocked prolongation takes parents as i / 2
     for (int l = 0; l + 1 < L_; l++)
-        if (cells(l) >= ((size_t)1 << 22)) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
+        if (cells(l) >= tb_min_cells_) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
     if (std::is_same<T, float>::value && consistent && tb_ && (nu1 == 1 || nu1 == 2) && (nu2 == 1 || nu2 == 2) &&
         L_ > 1 && even) {
         std::vector<T*> X = x_, B = b_;

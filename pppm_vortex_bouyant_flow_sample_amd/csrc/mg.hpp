@@ -66,9 +66,15 @@ class MGHier {
     // per workgroup: up to 4096 at 1024^3); the residual norm uses kResBlocks
     static constexpr int kNormBlocks = 4096;
     static constexpr int kResBlocks = 512;
-    // use the temporally blocked smoother (float only)
-    void set_blocked_smoother(bool on) { tb_ = on; }
-    bool blocked_smoother() const { return tb_; }
+    // float smoothing on large levels: 0 one launch per colour, 1 blocked
+    // passes (k_rb2), 2 colour-split blocked passes with a four-plane ring,
+    // 3 the same with a six-plane ring
+    void set_blocked_smoother(int kind) { tb_ = kind; }
+    int blocked_smoother() const { return tb_; }
+    // V-cycles run the blocked passes on levels with at least this many cells
+    // per component, one launch per colour below
+    void set_blocked_min_cells(size_t n) { tb_min_cells_ = n; }
+    size_t blocked_min_cells() const { return tb_min_cells_; }
 
    private:
     void release_locked();  // release() with runtime_mutex() already held
@@ -81,7 +87,8 @@ class MGHier {
     double* part_ = nullptr;      // norm partial sums
     double* hnorm_ = nullptr;     // pinned host copy of the reduced norms
     size_t bytes_ = 0;
-    bool tb_ = true;
+    int tb_ = 2;
+    size_t tb_min_cells_ = (size_t)1 << 18;
 };
 
 // FP64 defect-correction step of the mixed-precision solve:

@@ -204,24 +204,32 @@ extern "C" pppm_status pppm_scatter(const uint32_t* idx, const void* in, void* o
 // ---------------------------------------------------------------------------
 struct pppm_mg {
     int dtype = PPPM_F64;
+    int ncomp = 1;
     MGHier<float> f;
     MGHier<double> d;
 };
 
-extern "C" pppm_status pppm_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, pppm_mg** out) {
+extern "C" pppm_status pppm_mg_create_batched(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, int32_t ncomp,
+                                              pppm_mg** out) {
     if (!out) FAIL(PPPM_ERR_INVALID, "null out");
     *out = nullptr;
     if (nx < 1 || ny < 1 || nz < 1) FAIL(PPPM_ERR_INVALID, "grid dims");
     if (dtype != PPPM_F32 && dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "dtype");
+    if (ncomp < 1 || ncomp > 3) FAIL(PPPM_ERR_INVALID, "ncomp: 1..3");
     pppm_mg* m = new pppm_mg();
     m->dtype = dtype;
-    pppm_status s = dtype == PPPM_F32 ? m->f.init(nx, ny, nz, 1) : m->d.init(nx, ny, nz, 1);
+    m->ncomp = ncomp;
+    pppm_status s = dtype == PPPM_F32 ? m->f.init(nx, ny, nz, ncomp) : m->d.init(nx, ny, nz, ncomp);
     if (s != PPPM_OK) {
         delete m;
         return s;
     }
     *out = m;
     return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_mg_create(int32_t nx, int32_t ny, int32_t nz, int32_t dtype, pppm_mg** out) {
+    return pppm_mg_create_batched(nx, ny, nz, dtype, 1, out);
 }
 
 extern "C" void pppm_mg_destroy(pppm_mg* m) {
@@ -255,9 +263,9 @@ static pppm_status mg_solve_t(MGHier<T>& h, T* x, const T* b, int32_t schedule, 
             TRY(h.ref_fmg(x, b, (T)1, nu1, nu2, (T)omega, st));
         else
             TRY(h.vcycle(x, b, (T)1, nu1, nu2, (T)omega, false, st));
-        double rn, bn;
-        TRY(h.norms(x, b, (T)1, &rn, &bn, st));
-        rel[0] = bn > 0 ? rn / bn : rn;
+        double rn[3], bn[3];
+        TRY(h.norms(x, b, (T)1, rn, bn, st));
+        for (int c = 0; c < h.ncomp(); c++) rel[c] = bn[c] > 0 ? rn[c] / bn[c] : rn[c];
         cyc = 1;
         s = PPPM_OK;
     } else {
@@ -265,7 +273,7 @@ static pppm_status mg_solve_t(MGHier<T>& h, T* x, const T* b, int32_t schedule, 
         if (s != PPPM_OK && s != PPPM_ERR_NOT_CONVERGED) return s;
     }
     if (cycles_out) *cycles_out = cyc;
-    if (relres_out) *relres_out = rel[0];
+    if (relres_out) *relres_out = std::max(rel[0], std::max(rel[1], rel[2]));
     return s;
 }
 
@@ -287,10 +295,11 @@ extern "C" pppm_status pppm_mg_solve(pppm_mg* m, void* x, const void* b, int32_t
 extern "C" pppm_status pppm_mg_smooth(pppm_mg* m, void* x, const void* b, int32_t iters, double omega,
                                       int32_t blocked, void* stream) {
     if (!m || !x || !b || iters < 0) FAIL(PPPM_ERR_INVALID, "bad argument");
+    if (blocked < 0 || blocked > 3) FAIL(PPPM_ERR_INVALID, "blocked: 0..3");
     hipStream_t st = (hipStream_t)stream;
     if (m->dtype == PPPM_F32) {
-        bool prev = m->f.blocked_smoother();
-        m->f.set_blocked_smoother(blocked != 0);
+        int prev = m->f.blocked_smoother();
+        m->f.set_blocked_smoother(blocked);
         pppm_status s = m->f.smooth(0, (float*)x, (const float*)b, 1.0f, iters, (float)omega, 0.0f, st);
         m->f.set_blocked_smoother(prev);
         return s;
@@ -299,8 +308,15 @@ extern "C" pppm_status pppm_mg_smooth(pppm_mg* m, void* x, const void* b, int32_
     return m->d.smooth(0, (double*)x, (const double*)b, 1.0, iters, omega, 0.0, st);
 }
 
+extern "C" pppm_status pppm_mg_set_blocked_min_cells(pppm_mg* m, uint64_t cells) {
+    if (!m) FAIL(PPPM_ERR_INVALID, "null mg");
+    m->f.set_blocked_min_cells((size_t)cells);
+    return PPPM_OK;
+}
+
 extern "C" pppm_status pppm_mg_set_blocked(pppm_mg* m, int32_t blocked) {
     if (!m) FAIL(PPPM_ERR_INVALID, "null mg");
-    m->f.set_blocked_smoother(blocked != 0);
+    if (blocked < 0 || blocked > 3) FAIL(PPPM_ERR_INVALID, "blocked: 0..3");
+    m->f.set_blocked_smoother(blocked);
     return PPPM_OK;
 }

@@ -429,12 +429,16 @@ class Hash:
 class MG:
     """pppm_mg: standalone multigrid on caller grids."""
 
-    def __init__(self, dims, dtype=F64):
+    def __init__(self, dims, dtype=F64, ncomp=1):
         self.h = C.c_void_p()
         self.dims = tuple(int(d) for d in dims)
         self.dtype = dtype
-        check(lib().pppm_mg_create(*[C.c_int32(d) for d in self.dims], C.c_int32(dtype), C.byref(self.h)),
-              "mg_create")
+        self.ncomp = int(ncomp)
+        check(lib().pppm_mg_create_batched(*[C.c_int32(d) for d in self.dims], C.c_int32(dtype),
+                                           C.c_int32(self.ncomp), C.byref(self.h)), "mg_create")
+
+    def set_blocked(self, kind):
+        check(lib().pppm_mg_set_blocked(self.h, C.c_int32(kind)), "mg_set_blocked")
 
     def levels(self):
         L = C.c_int32()

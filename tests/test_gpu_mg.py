@@ -62,20 +62,24 @@ def test_mg_fp32_reaches_working_precision(native, oracle):
 
 @pytest.mark.parametrize("dims,iters", [((64, 64, 64), 2), ((48, 40, 72), 1), ((256, 32, 20), 3), ((9, 7, 5), 2)])
 def test_blocked_smoother_equals_halfsweep_kernel(native, dims, iters):
-    """The temporally blocked LDS smoother performs exactly the red-black
-    half-sweeps of the one-launch-per-colour kernel (FP32, rounding only)."""
+    """The temporally blocked LDS smoothers (kind 1, and the colour-split
+    kinds 2 and 3, which take kind 1's kernel for odd nx) perform exactly the
+    red-black half-sweeps of the one-launch-per-colour kernel (FP32, rounding
+    only)."""
     n = int(np.prod(dims))
     rng = np.random.default_rng(sum(dims))
     x0 = rng.standard_normal(n).astype(np.float32)
     b = rng.standard_normal(n).astype(np.float32)
     mg = native.MG(dims, native.F32)
     out = []
-    for blocked in (0, 1):
+    for blocked in (0, 1, 2, 3):
+        print(f"[mg] smooth dims={dims} iters={iters} kind={blocked}", flush=True)
         bx, bb = native.DeviceBuffer.from_array(x0), native.DeviceBuffer.from_array(b)
         native.check(native.lib().pppm_mg_smooth(mg.h, bx.ptr, bb.ptr, iters, native.C.c_double(1.0), blocked, None))
         native.synchronize()
         out.append(bx.to_array(np.float32, n))
-    assert np.max(np.abs(out[0] - out[1])) <= 1e-5 * np.max(np.abs(out[0]))
+    for q in range(1, 4):
+        assert np.max(np.abs(out[0] - out[q])) <= 1e-5 * np.max(np.abs(out[0])), q
 
 
 @pytest.mark.parametrize("dims,nu", [((256, 128, 128), (2, 2)), ((256, 256, 64), (1, 2)), ((192, 160, 144), (2, 1)),
@@ -87,11 +91,39 @@ def test_fused_vcycle_equals_unfused(native, oracle, dims, nu):
     xs, b = _manufactured(oracle, dims, 9)
     b = b.astype(np.float32)
     res = []
-    for blocked in (0, 1):
+    for blocked in (0, 1, 2, 3):
+        print(f"[mg] vcycle dims={dims} nu={nu} kind={blocked}", flush=True)
         mg = native.MG(dims, native.F32)
-        native.check(native.lib().pppm_mg_set_blocked(mg.h, blocked))
+        mg.set_blocked(blocked)
         bx, bb = native.DeviceBuffer(4 * b.size), native.DeviceBuffer.from_array(b)
         bx.fill_zero()
         mg.solve(bx.ptr, bb.ptr, native.MG_VCYCLE, nu[0], nu[1], 1.0, 0.0, 2)
         res.append(bx.to_array(np.float32, b.size))
-    assert rel_l2(res[1], res[0]) < 1e-5
+    for q in range(1, 4):
+        assert rel_l2(res[q], res[0]) < 1e-5, q
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+def test_batched_components_equal_single(native, oracle, kind):
+    """Three components solved together (one launch per pass over all of
+    them) give each component's single solve, to rounding."""
+    dims = (256, 128, 128)
+    n = int(np.prod(dims))
+    bs = [_manufactured(oracle, dims, 20 + c)[1].astype(np.float32) for c in range(3)]
+    print(f"[mg] batched dims={dims} kind={kind}", flush=True)
+    mg3 = native.MG(dims, native.F32, ncomp=3)
+    mg3.set_blocked(kind)
+    bx, bb = native.DeviceBuffer(12 * n), native.DeviceBuffer.from_array(np.concatenate(bs))
+    bx.fill_zero()
+    cyc3, rel3, _ = mg3.solve(bx.ptr, bb.ptr, native.MG_VCYCLE, 2, 2, 1.0, 0.0, 3)
+    x3 = bx.to_array(np.float32, 3 * n).reshape(3, n)
+    worst = 0.0
+    for c in range(3):
+        mg = native.MG(dims, native.F32)
+        mg.set_blocked(kind)
+        bx1, bb1 = native.DeviceBuffer(4 * n), native.DeviceBuffer.from_array(bs[c])
+        bx1.fill_zero()
+        _, rel1, _ = mg.solve(bx1.ptr, bb1.ptr, native.MG_VCYCLE, 2, 2, 1.0, 0.0, 3)
+        assert rel_l2(x3[c], bx1.to_array(np.float32, n)) < 1e-6, c
+        worst = max(worst, rel1)
+    assert abs(rel3 - worst) <= 1e-3 * worst
