@@ -208,9 +208,10 @@ struct pppm_ctx {
     bool use_graphs = true;
     hipGraphExec_t g_mixed[2] = {nullptr, nullptr};
     hipGraphExec_t g_plain = nullptr;
+    hipGraphExec_t g_warm = nullptr;
 
     void drop_graphs() {
-        for (hipGraphExec_t* e : {&g_mixed[0], &g_mixed[1], &g_plain})
+        for (hipGraphExec_t* e : {&g_mixed[0], &g_mixed[1], &g_plain, &g_warm})
             if (*e) {
                 (void)hipGraphExecDestroy(*e);
                 *e = nullptr;
@@ -519,6 +520,9 @@ extern "C" pppm_status pppm_set_vortices(pppm_ctx* c, uint32_t n, const float* p
     return PPPM_OK;
 }
 
+// FP32 V-cycles before the first FP64 defect correction of a mixed solve.
+constexpr int kMixedWarmCycles = 4;
+
 static pppm_status solve_mixed(pppm_ctx* c) {
     hipStream_t st = c->st;
     const int g = (int)c->g;
@@ -538,14 +542,37 @@ static pppm_status solve_mixed(pppm_ctx* c) {
     };
     double rel[4];
     c->psi_cur = 0;
+    // r32 = b (psi = 0) and the norms of b
     TRY(launch_dc_step(nullptr, nullptr, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn, st));
     TRY(read_rel(rel));
     int outer = 0, inner = 0;
+    // Warm start: the first V-cycles run in FP32 in place on e (A e = r32 from
+    // e = 0), as far as they contract the residual before FP32 rounding stalls
+    // them (about 2e-5 after four V(2,2) cycles, against a floor near 1e-6);
+    // then psi = e and its FP64 defect.  Each later outer step is one V-cycle
+    // on the FP64 defect.
+    const int warm = std::min(kMixedWarmCycles, c->p.mg_max_cycles);
+    auto warmup = [c, g, outn, st, warm]() -> pppm_status {
+        for (int q = 0; q < warm; q++)
+            TRY(c->mg.vcycle(c->e32, c->r32, 1.0f, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st, q == 0));
+        return launch_dc_step(nullptr, c->e32, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn,
+                              st);
+    };
+    if (warm > 0 && rel[3] > c->p.mg_tol) {
+        if (c->use_graphs) {
+            if (!c->g_warm) TRY(c->capture(&c->g_warm, warmup));
+            PPPM_HIP_TRY(hipGraphLaunch(c->g_warm, st));
+        } else {
+            TRY(warmup());
+        }
+        inner += warm;
+        outer++;
+        TRY(read_rel(rel));
+    }
     // one outer step: e = V-cycle(A e = r32, e = 0); psi_next = psi + e;
     // r32 = b - A psi_next and its FP64 norms
     auto step = [c, g, outn, st](int cur) -> pppm_status {
-        TRY(c->mg.fill(c->e32, 3 * c->G, st));
-        TRY(c->mg.vcycle(c->e32, c->r32, 1.0f, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st));
+        TRY(c->mg.vcycle(c->e32, c->r32, 1.0f, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st, true));
         return launch_dc_step(c->psi64[cur], c->e32, c->b64, c->psi64[1 - cur], c->r32, g, g, g, 3,
                               c->mg.partials(), outn, st);
     };

@@ -1407,12 +1407,12 @@ static pppm_status vcycle_from(MGHier<T>& m, std::vector<T*>& X, std::vector<T*>
 // restriction, and nu2 passes on the way up, the first adding the prolonged
 // coarse correction as it loads x.
 template <typename T>
-static pppm_status vcycle_fused(MGHier<T>&, std::vector<T*>&, std::vector<T*>&, T, int, int, T, hipStream_t) {
+static pppm_status vcycle_fused(MGHier<T>&, std::vector<T*>&, std::vector<T*>&, T, int, int, T, bool, hipStream_t) {
     return PPPM_ERR_INVALID;
 }
 template <>
 pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::vector<float*>& B, float bscale0,
-                                int nu1, int nu2, float omega, hipStream_t st) {
+                                int nu1, int nu2, float omega, bool x0_zero, hipStream_t st) {
     const int L = m.levels(), nc = m.ncomp(), kind = m.blocked_smoother();
     pppm_status s;
     auto level_args = [&](int l) {
@@ -1444,7 +1444,7 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
     for (int l = 0; l < L - 1; l++) {
         if (!blocked(l)) {
             const float bs = l == 0 ? bscale0 : 1.0f, beta = (float)mg_beta(l);
-            if (l > 0 && (s = m.fill(X[l], m.cells(l) * nc, st)) != PPPM_OK) return s;
+            if ((l > 0 || x0_zero) && (s = m.fill(X[l], m.cells(l) * nc, st)) != PPPM_OK) return s;
             if ((s = m.smooth_colour(l, X[l], B[l], bs, nu1, omega, beta, st)) != PPPM_OK) return s;
             if ((s = m.residual_restrict(l, X[l], B[l], bs, beta, B[l + 1], st)) != PPPM_OK) return s;
             cur[l] = X[l];
@@ -1457,7 +1457,7 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
             a.xin = src;
             a.x = dst;
             a.bn = B[l + 1];
-            const bool zero = l > 0 && q == 0, last = q == nu1 - 1;
+            const bool zero = (l > 0 || x0_zero) && q == 0, last = q == nu1 - 1;
             if (last)
                 s = zero ? launch_rb2<true, false, true>(a, nc, kind, st) : launch_rb2<true, false, false>(a, nc, kind, st);
             else
@@ -1498,7 +1498,7 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
 
 template <typename T>
 pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent,
-                              hipStream_t st) {
+                              hipStream_t st, bool x0_zero) {
     bool even = true;  // the blocked prolongation takes parents as i / 2
     for (int l = 0; l + 1 < L_; l++)
         if (cells(l) >= tb_min_cells_) even = even && !(d_[l][0] & 1) && !(d_[l][1] & 1) && !(d_[l][2] & 1);
@@ -1507,11 +1507,15 @@ pppm_status MGHier<T>::vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T
         std::vector<T*> X = x_, B = b_;
         X[0] = x0;
         B[0] = const_cast<T*>(b0);
-        return vcycle_fused<T>(*this, X, B, bscale0, nu1, nu2, omega, st);
+        return vcycle_fused<T>(*this, X, B, bscale0, nu1, nu2, omega, x0_zero, st);
     }
     std::vector<T*> X = x_, B = b_;
     X[0] = x0;
     B[0] = const_cast<T*>(b0);
+    if (x0_zero) {
+        pppm_status s = fill(x0, cells(0) * ncomp_, st);
+        if (s != PPPM_OK) return s;
+    }
     return vcycle_from(*this, X, B, 0, bscale0, nu1, nu2, omega, consistent, st);
 }
 

@@ -36,7 +36,10 @@ class MGHier {
     // consistent: trilinear prolongation and boundary-consistent coarse
     // operators (tolerance-driven cycle); otherwise the reference pair
     // (piecewise-constant prolongation, zero ghosts on every level).
-    pppm_status vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent, hipStream_t st);
+    // x0_zero: x0 holds no starting value (treated as zero; its contents are
+    // not read)
+    pppm_status vcycle(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, bool consistent, hipStream_t st,
+                       bool x0_zero = false);
     // MultiGrid.cpp:202-250 (m_FullMultiGrid): one FMG pass plus one V-cycle.
     pppm_status ref_fmg(T* x0, const T* b0, T bscale0, int nu1, int nu2, T omega, hipStream_t st);
     // ||b*bscale - A x||_2 and ||b*bscale||_2 per component (FP64 sums), synchronous.

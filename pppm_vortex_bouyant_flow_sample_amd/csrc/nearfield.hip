@@ -58,17 +58,15 @@ __device__ inline int near_hi_cell(float v, double rb, double invh, int gi) {
 // Cluster form: one wave takes kNearTI consecutive sorted evaluation points
 // (usually one hash cell) and splits the vortices among kNearSJ lanes per
 // point (lane = point + kNearTI * slice).  It scans the hash rows covering the
-// union of the points' boxes; each row segment (consecutive x cells are
-// consecutive sorted vortices) is staged 64 vortices at a time into LDS with
-// coalesced loads, and each lane tests its slice of the chunk against its
-// point, evaluating the kernel where the box test passes.  The row ranges of
-// the box are read in one vector load (a row per lane) and the chunks run as
-// one stream across rows, the next chunk's loads in flight while the current
-// one is tested.  The M2P corners of
-// a point are shared among its lanes; partial sums are reduced over the
-// slices at the end (fixed order, deterministic).  All loop bounds are
-// wave-uniform.
-constexpr int kNearTI = 4;
+// union of the points' boxes (consecutive x cells of a row are consecutive
+// sorted vortices).  The rows' vortex ranges are read in one vector load (a
+// row per lane) and packed into one stream, which is staged 64 vortices at a
+// time into LDS with the next chunk's loads in flight; each lane tests its
+// slice of the chunk against its point and evaluates the kernel where the box
+// test passes.  The M2P corners of a point are shared among its lanes;
+// partial sums are reduced over the slices at the end (fixed order,
+// deterministic).  All loop bounds are wave-uniform.
+constexpr int kNearTI = 2;
 constexpr int kNearSJ = 64 / kNearTI;
 
 template <int KERNEL, bool SEGMENT, typename TO>
@@ -148,42 +146,57 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                 lp1 = min(off[rowc + xhi + 1], nsrc);
             }
             const int rn = min(64, nrows - r0);
-            // chunk cursor (row cr, vortices [cb, min(cb + 64, ce))), wave-uniform
-            auto advance = [&](int& r, uint32_t& b, uint32_t& e) {
-                b += 64;
-                while (b >= e && r < rn) {
-                    r++;
-                    if (r < rn) {
-                        b = __builtin_amdgcn_readlane(lp0, r);
-                        e = __builtin_amdgcn_readlane(lp1, r);
-                    }
-                }
-            };
-            int cr = 0;
-            uint32_t cb = __builtin_amdgcn_readlane(lp0, 0), ce = __builtin_amdgcn_readlane(lp1, 0);
-            if (cb >= ce) {
-                cb -= 64;  // wraps; advance adds it back and skips the empty rows
-                advance(cr, cb, ce);
+            // The rows are packed into one stream of vortices: chunk c holds
+            // stream positions 64 c .. 64 c + 63 across row boundaries.  pre
+            // is a row's first stream position (exclusive prefix of the row
+            // lengths), roff maps stream position to vortex index.
+            const uint32_t len = lp1 > lp0 ? lp1 - lp0 : 0u;
+            uint32_t incl = len;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t up = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += up;
             }
+            const uint32_t pre = incl - len, roff = lp0 - pre;
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+            // vortex index of this lane's position in the chunk at S0; rcur
+            // (wave-uniform) is a row at or before the one holding S0 and is
+            // advanced to the last row starting at or before S0 + 64
+            auto locate = [&](uint32_t S0, int& rcur) {
+                const uint32_t S = S0 + (uint32_t)lane;
+                uint32_t mo = __builtin_amdgcn_readlane(roff, rcur);
+                int rnext = rcur;
+                for (int rr = rcur + 1; rr < rn; rr++) {
+                    const uint32_t p = __builtin_amdgcn_readlane(pre, rr);
+                    if (p > S0 + 64) break;
+                    rnext = rr;
+                    if (S >= p) mo = __builtin_amdgcn_readlane(roff, rr);
+                }
+                rcur = rnext;
+                return S + mo;
+            };
+            int rc = 0;
             // the next chunk is loaded into registers while this one is tested
             float4 pp = make_float4(0.f, 0.f, 0.f, 0.f), po = pp;
-            if (cr < rn && cb + lane < ce) {
-                pp = spos[cb + lane];
-                po = somg[cb + lane];
+            if (total > 0) {
+                const uint32_t vi = locate(0u, rc);
+                if ((uint32_t)lane < total) {
+                    pp = spos[vi];
+                    po = somg[vi];
+                }
             }
-            while (cr < rn) {
-                const uint32_t cnt = min(64u, ce - cb);
+            for (uint32_t S0 = 0; S0 < total; S0 += 64) {
+                const uint32_t cnt = min(64u, total - S0);
                 cpos[w][lane] = pp;
                 comg[w][lane] = po;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                int nr = cr;
-                uint32_t nb = cb, ne = ce;
-                advance(nr, nb, ne);
-                if (nr < rn && nb + lane < ne) {
-                    pp = spos[nb + lane];
-                    po = somg[nb + lane];
+                if (S0 + 64 < total) {
+                    const uint32_t vi = locate(S0 + 64, rc);
+                    if (S0 + 64 + (uint32_t)lane < total) {
+                        pp = spos[vi];
+                        po = somg[vi];
+                    }
                 }
 #pragma unroll
                 for (int qq = 0; qq < kNearTI; qq++) {
@@ -212,9 +225,6 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                cr = nr;
-                cb = nb;
-                ce = ne;
             }
         }
     }

@@ -158,3 +158,4 @@ def test_graph_replay_equals_eager(native, prec):
     eager.update_params(p2)
     assert np.array_equal(native.solve_self(ctx, scene), native.solve_self(eager, scene))
     assert ctx.device_bytes() > 0
+
