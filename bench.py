@@ -174,7 +174,7 @@ def near_traffic():
     """HBM bytes per launch of the near-field kernel on this workload, from the
     committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled
     by the same counter's reading on a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "near_field_plume8M_256_counters.txt")
+    path = os.path.join(ROOT, "profiles", "hbm_counters_plume8M_256.md")
     if not os.path.exists(path):
         return None
     vals = {}
