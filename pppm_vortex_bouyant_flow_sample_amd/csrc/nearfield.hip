@@ -11,21 +11,26 @@ struct NearConst {
     float inv_a;       // 1 / core radius
     float inv_a3;      // 1 / a^3
     float inv4pi;      // 1 / (4 pi) with the reference's 4 pi
+    float kexp2;       // -log2(e) / a^3: exp(-(r/a)^3) = exp2(kexp2 r^3)
+    float r3_series;   // 0.1 a^3: below it (x = (r/a)^3 < 0.1) the series form
+    float sc[5];       // series coefficients of (1 - e^-x) / x, times 1 / (4 pi)
 };
 
 // q(rho) / (4 pi r^3) for the expcubic core, q = 1 - exp(-(r/a)^3)
-// (GPU_kernel.cu:2389-2402), small-x series to avoid cancellation.
+// (GPU_kernel.cu:2389-2402), with a series for x = (r/a)^3 < 0.1 to avoid
+// cancellation.  1 / (4 pi) is folded into q (q' = q / (4 pi)), so the
+// coefficient is rinv^3 q'.
 __device__ inline float coeff_expcubic(float r2, const NearConst& nc) {
-    float rinv = __frsqrt_rn(r2);
-    float r = r2 * rinv;
-    float x = r2 * r * nc.inv_a3;
+    const float rinv = __frsqrt_rn(r2);
+    const float r3 = r2 * (r2 * rinv);
     float q;
-    if (x < 0.1f) {
-        q = x * (1.0f + x * (-0.5f + x * (1.0f / 6.0f + x * (-1.0f / 24.0f + x * (1.0f / 120.0f)))));
+    if (r3 < nc.r3_series) {
+        const float x = r3 * nc.inv_a3;
+        q = x * (nc.sc[0] + x * (nc.sc[1] + x * (nc.sc[2] + x * (nc.sc[3] + x * nc.sc[4]))));
     } else {
-        q = 1.0f - __expf(-x);
+        q = fmaf(-nc.inv4pi, __builtin_amdgcn_exp2f(nc.kexp2 * r3), nc.inv4pi);
     }
-    return q * (rinv * rinv * rinv) * nc.inv4pi;
+    return q * (rinv * rinv * rinv);
 }
 
 // Gaussian core: q = erf(rho/sqrt2) - sqrt(2/pi) rho exp(-rho^2/2)
@@ -175,8 +180,11 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                 return S + mo;
             };
             int rc = 0;
-            // the next chunk is loaded into registers while this one is tested
-            float4 pp = make_float4(0.f, 0.f, 0.f, 0.f), po = pp;
+            // The next chunk is loaded into registers while this one is
+            // tested.  Slots past the end of the stream hold a position far
+            // outside every box, so the box test rejects them unasked.
+            const float4 far_away = make_float4(3.0e30f, 3.0e30f, 3.0e30f, 0.f);
+            float4 pp = far_away, po = make_float4(0.f, 0.f, 0.f, 0.f);
             if (total > 0) {
                 const uint32_t vi = locate(0u, rc);
                 if ((uint32_t)lane < total) {
@@ -185,12 +193,12 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                 }
             }
             for (uint32_t S0 = 0; S0 < total; S0 += 64) {
-                const uint32_t cnt = min(64u, total - S0);
                 cpos[w][lane] = pp;
                 comg[w][lane] = po;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                pp = far_away;
                 if (S0 + 64 < total) {
                     const uint32_t vi = locate(S0 + 64, rc);
                     if (S0 + 64 + (uint32_t)lane < total) {
@@ -201,24 +209,24 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
 #pragma unroll
                 for (int qq = 0; qq < kNearTI; qq++) {
                     const uint32_t q = sj + qq * kNearSJ;
-                    if (q < cnt) {
-                        const float4 s = cpos[w][q];
-                        const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
-                        bool acc = tv && fabsf(dx) <= rbox && fabsf(dy) <= rbox && fabsf(dz) <= rbox;
-                        const float r2 = dx * dx + dy * dy + dz * dz;
-                        acc = acc && r2 >= 1e-24f;
-                        if (SEGMENT) {
-                            const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
-                            acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
-                        }
-                        if (acc) {
-                            const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
-                                                                            : coeff_expcubic(r2, nc);
-                            const float4 o = comg[w][q];
-                            vx += (o.y * dz - o.z * dy) * cf;
-                            vy += (o.z * dx - o.x * dz) * cf;
-                            vz += (o.x * dy - o.y * dx) * cf;
-                        }
+                    const float4 s = cpos[w][q];
+                    const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    // |d|inf <= rbox as one max3 of absolute values; a point
+                    // past the end of the evaluation set (tv false) computes
+                    // on a copy of its cluster's first point and is not stored
+                    bool acc = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) <= rbox && r2 >= 1e-24f;
+                    if (SEGMENT) {
+                        const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
+                        acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
+                    }
+                    if (acc) {
+                        const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
+                                                                        : coeff_expcubic(r2, nc);
+                        const float4 o = comg[w][q];
+                        vx += (o.y * dz - o.z * dy) * cf;
+                        vy += (o.z * dx - o.x * dz) * cf;
+                        vz += (o.x * dy - o.y * dx) * cf;
                     }
                 }
                 // the next chunk overwrites the staging slots
@@ -277,6 +285,12 @@ pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st) {
     nc.inv_a = (float)(1.0 / a.core_a);
     nc.inv_a3 = (float)(1.0 / (a.core_a * a.core_a * a.core_a));
     nc.inv4pi = (float)(1.0 / kRefFourPi);
+    nc.kexp2 = (float)(-1.4426950408889634 / (a.core_a * a.core_a * a.core_a));
+    nc.r3_series = (float)(0.1 * a.core_a * a.core_a * a.core_a);
+    {
+        const double c[5] = {1.0, -0.5, 1.0 / 6.0, -1.0 / 24.0, 1.0 / 120.0};
+        for (int k = 0; k < 5; k++) nc.sc[k] = (float)(c[k] / kRefFourPi);
+    }
     (void)h;
     const bool seg = a.eoth != nullptr;
     if (a.kernel == PPPM_KERNEL_GAUSSIAN) {
