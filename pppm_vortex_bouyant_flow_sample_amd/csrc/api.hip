@@ -542,23 +542,23 @@ static pppm_status solve_mixed(pppm_ctx* c) {
     };
     double rel[4];
     c->psi_cur = 0;
-    // r32 = b (psi = 0) and the norms of b
-    TRY(launch_dc_step(nullptr, nullptr, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn, st));
-    TRY(read_rel(rel));
     int outer = 0, inner = 0;
-    // Warm start: the first V-cycles run in FP32 in place on e (A e = r32 from
-    // e = 0), as far as they contract the residual before FP32 rounding stalls
-    // them (about 2e-5 after four V(2,2) cycles, against a floor near 1e-6);
-    // then psi = e and its FP64 defect.  Each later outer step is one V-cycle
-    // on the FP64 defect.
+    // Warm start: the first V-cycles run in FP32 in place on e from e = 0, on
+    // the FP32 vorticity grid with the scale -h^2 (the FP32 path's right-hand
+    // side, equal to (float)b to FP32 rounding), as far as they contract the
+    // residual before FP32 rounding stalls them (about 2e-5 after four V(2,2)
+    // cycles, against a floor near 1e-6); then psi = e with its FP64 defect
+    // r = b - A psi and the FP64 norms of r and b.  Each later outer step is
+    // one V-cycle on the FP64 defect.
     const int warm = std::min(kMixedWarmCycles, c->p.mg_max_cycles);
-    auto warmup = [c, g, outn, st, warm]() -> pppm_status {
+    const float bs = (float)(-c->pg.h * c->pg.h);
+    auto warmup = [c, g, outn, st, warm, bs]() -> pppm_status {
         for (int q = 0; q < warm; q++)
-            TRY(c->mg.vcycle(c->e32, c->r32, 1.0f, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st, q == 0));
+            TRY(c->mg.vcycle(c->e32, c->wg, bs, c->p.nu1, c->p.nu2, (float)c->p.mg_omega, true, st, q == 0));
         return launch_dc_step(nullptr, c->e32, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn,
                               st);
     };
-    if (warm > 0 && rel[3] > c->p.mg_tol) {
+    if (warm > 0) {
         if (c->use_graphs) {
             if (!c->g_warm) TRY(c->capture(&c->g_warm, warmup));
             PPPM_HIP_TRY(hipGraphLaunch(c->g_warm, st));
@@ -567,8 +567,11 @@ static pppm_status solve_mixed(pppm_ctx* c) {
         }
         inner += warm;
         outer++;
-        TRY(read_rel(rel));
+    } else {
+        // r32 = b (psi = 0) and the norms of b
+        TRY(launch_dc_step(nullptr, nullptr, c->b64, c->psi64[0], c->r32, g, g, g, 3, c->mg.partials(), outn, st));
     }
+    TRY(read_rel(rel));
     // one outer step: e = V-cycle(A e = r32, e = 0); psi_next = psi + e;
     // r32 = b - A psi_next and its FP64 norms
     auto step = [c, g, outn, st](int cur) -> pppm_status {

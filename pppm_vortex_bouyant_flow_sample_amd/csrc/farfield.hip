@@ -20,8 +20,8 @@ __device__ inline double ld(const TP* p, size_t i) {
 // A workgroup owns a 32 x 8 tile of cells and a run of z planes.  The grid
 // vorticity streams through an LDS ring of 2K+1 planes with a K-cell halo
 // (zero outside the grid, which is the reference's in-grid-only sum), so the
-// (2K+1)^3 local correction reads LDS instead of global memory; the curl of
-// psi and of the diagonal term read global memory (12 + 12 loads).
+// (2K+1)^3 local correction and the curl of the diagonal term read LDS; the
+// curl of psi reads global memory (12 loads, mostly L2 hits).
 constexpr int kCfTX = 32, kCfTY = 8, kCfZC = 16;
 
 template <typename TP, typename TC, int K>
@@ -78,30 +78,52 @@ __global__ void __launch_bounds__(256) k_curl_far(const TP* __restrict__ psi, co
                                  (TC)Pz[at(ip, j, k)]);
             TC uz = half_invh * ((TC)Py[at(ip, j, k)] - (TC)Py[at(im, j, k)] + (TC)Px[at(i, jm, k)] -
                                  (TC)Px[at(i, jp, k)]);
-            // diagonal term: curl of 0.2505806 h^2 w with the same clamped stencil
-            float dx = cdiag_half_invh * (Wz[at(i, jp, k)] - Wz[at(i, jm, k)] + Wy[at(i, j, km)] - Wy[at(i, j, kp)]);
-            float dy = cdiag_half_invh * (Wx[at(i, j, kp)] - Wx[at(i, j, km)] + Wz[at(im, j, k)] - Wz[at(ip, j, k)]);
-            float dz = cdiag_half_invh * (Wy[at(ip, j, k)] - Wy[at(im, j, k)] + Wx[at(i, jm, k)] - Wx[at(i, jp, k)]);
-            // local correction over the (2K+1)^3 block, same order as the reference loop
+            // diagonal term: curl of 0.2505806 h^2 w with the same clamped
+            // stencil; for K >= 1 the ring holds the neighbours (the clamp
+            // reads the edge cell itself, which the ring holds too)
+            float dx, dy, dz;
+            if (K >= 1) {
+                auto wr = [&](int comp, int ii, int jj, int kk) {
+                    return ring[((kk % W) + W) % W][comp][(jj - gy0) * LXW + (ii - gx0)];
+                };
+                dx = cdiag_half_invh * (wr(2, i, jp, k) - wr(2, i, jm, k) + wr(1, i, j, km) - wr(1, i, j, kp));
+                dy = cdiag_half_invh * (wr(0, i, j, kp) - wr(0, i, j, km) + wr(2, im, j, k) - wr(2, ip, j, k));
+                dz = cdiag_half_invh * (wr(1, ip, j, k) - wr(1, im, j, k) + wr(0, i, jm, k) - wr(0, i, jp, k));
+            } else {
+                dx = cdiag_half_invh * (Wz[at(i, jp, k)] - Wz[at(i, jm, k)] + Wy[at(i, j, km)] - Wy[at(i, j, kp)]);
+                dy = cdiag_half_invh * (Wx[at(i, j, kp)] - Wx[at(i, j, km)] + Wz[at(im, j, k)] - Wz[at(ip, j, k)]);
+                dz = cdiag_half_invh * (Wy[at(ip, j, k)] - Wy[at(im, j, k)] + Wx[at(i, jm, k)] - Wx[at(i, jp, k)]);
+            }
+            // local correction over the (2K+1)^3 block.  dir is odd,
+            // dir[-D] = -dir[D] (a central difference of the even 1/(4 pi |r|),
+            // computed so in FP64 and rounded), so D and -D are taken
+            // together: dir[D] x (w[c+D] - w[c-D]); h^3 is applied once.
             float vx = 0.f, vy = 0.f, vz = 0.f;
 #pragma unroll
-            for (int a = -K; a <= K; a++) {
+            for (int a = 0; a <= K; a++) {
                 const float* rp = &ring[(((k + a) % W) + W) % W][0][0];
+                const float* rm = &ring[(((k - a) % W) + W) % W][0][0];
 #pragma unroll
                 for (int b = -K; b <= K; b++) {
+                    if (a == 0 && b < 0) continue;
                     const float* drow = dir + 3 * (size_t)(((a + K) * W + (b + K)) * W);
-                    const int row = (ly + K + b) * LXW + lx + K;
+                    const int rowp = (ly + K + b) * LXW + lx + K, rowm = (ly + K - b) * LXW + lx + K;
 #pragma unroll
                     for (int e = -K; e <= K; e++) {
-                        if (a == 0 && b == 0 && e == 0) continue;
+                        if (a == 0 && b == 0 && e <= 0) continue;
                         const float d0 = drow[3 * (e + K)], d1 = drow[3 * (e + K) + 1], d2 = drow[3 * (e + K) + 2];
-                        const float ox = rp[row + e] * h3, oy = rp[PLW + row + e] * h3, oz = rp[2 * PLW + row + e] * h3;
+                        const float ox = rp[rowp + e] - rm[rowm - e];
+                        const float oy = rp[PLW + rowp + e] - rm[PLW + rowm - e];
+                        const float oz = rp[2 * PLW + rowp + e] - rm[2 * PLW + rowm - e];
                         vx += d1 * oz - d2 * oy;
                         vy += d2 * ox - d0 * oz;
                         vz += d0 * oy - d1 * ox;
                     }
                 }
             }
+            vx *= h3;
+            vy *= h3;
+            vz *= h3;
             const size_t c = at(i, j, k);
             far[c] = (float)((double)ux - (double)vx - (double)dx);
             far[(size_t)G + c] = (float)((double)uy - (double)vy - (double)dy);
