@@ -674,9 +674,10 @@ extern "C" pppm_status pppm_compute_far_field(pppm_ctx* c) {
 }
 
 static pppm_status near_common(pppm_ctx* c, uint32_t m, const float4* epos, const float4* eoth,
-                               const uint32_t* eidx, void* ux, void* uy, void* uz, uint32_t u_stride,
-                               int32_t u_dtype) {
+                               const uint32_t* eidx, const uint32_t* tperm, void* ux, void* uy, void* uz,
+                               uint32_t u_stride, int32_t u_dtype) {
     NearArgs a;
+    a.tperm = tperm;
     a.epos = epos;
     a.eoth = eoth;
     a.eidx = eidx;
@@ -706,7 +707,10 @@ extern "C" pppm_status pppm_evaluate_self(pppm_ctx* c, void* ux, void* uy, void*
     if (u_dtype != PPPM_F32 && u_dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "u_dtype");
     if (u_stride == 0) FAIL(PPPM_ERR_INVALID, "u_stride");
     c->stats.m = c->n;
-    return near_common(c, c->n, c->spos, nullptr, c->vs.vs, ux, uy, uz, u_stride, u_dtype);
+    // octant order within cells, into the sort's spare index buffer
+    uint32_t* perm = c->vs.vs == c->vs.v0 ? c->vs.v1 : c->vs.v0;
+    TRY(launch_octant_order(c->vs.ks, c->spos, c->n, c->pg.invh, perm, c->st));
+    return near_common(c, c->n, c->spos, nullptr, c->vs.vs, perm, ux, uy, uz, u_stride, u_dtype);
 }
 
 extern "C" pppm_status pppm_evaluate(pppm_ctx* c, uint32_t m, const float* eval_pos4, const float* other_end4,
@@ -725,7 +729,10 @@ extern "C" pppm_status pppm_evaluate(pppm_ctx* c, uint32_t m, const float* eval_
     TRY(launch_reorder_eval(c->es.vs, (const float4*)eval_pos4, (const float4*)other_end4, m, c->pg, c->epos,
                             other_end4 ? c->eoth : nullptr, st));
     c->stats.m = m;
-    return near_common(c, m, c->epos, other_end4 ? c->eoth : nullptr, c->es.vs, ux, uy, uz, u_stride, u_dtype);
+    uint32_t* perm = c->es.vs == c->es.v0 ? c->es.v1 : c->es.v0;
+    TRY(launch_octant_order(c->es.ks, c->epos, m, c->pg.invh, perm, st));
+    return near_common(c, m, c->epos, other_end4 ? c->eoth : nullptr, c->es.vs, perm, ux, uy, uz, u_stride,
+                       u_dtype);
 }
 
 extern "C" pppm_status pppm_solve_self(pppm_ctx* c, uint32_t n, const float* pos4, const void* wx, const void* wy,

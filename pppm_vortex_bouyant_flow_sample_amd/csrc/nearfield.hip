@@ -80,15 +80,18 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                                                   const uint32_t* __restrict__ off, const float4* __restrict__ spos,
                                                   const float4* __restrict__ somg, const float* __restrict__ far,
                                                   uint32_t g, int K, double invh, NearConst nc, TO* __restrict__ ux,
-                                                  TO* __restrict__ uy, TO* __restrict__ uz, uint32_t u_stride) {
+                                                  TO* __restrict__ uy, TO* __restrict__ uz, uint32_t u_stride,
+                                                  const uint32_t* __restrict__ tperm) {
     __shared__ float4 cpos[4][64], comg[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ti = lane % kNearTI, sj = lane / kNearTI;
     const uint32_t t0 = (blockIdx.x * 4 + w) * kNearTI;
     if (t0 >= m) return;  // the whole wave leaves together; no block barriers below
-    const uint32_t t = t0 + ti;
-    const bool tv = t < m;
-    const float4 a = epos[tv ? t : t0];
+    // evaluation slot -> sorted point (octant order within cells, or itself)
+    const uint32_t slot = t0 + ti;
+    const bool tv = slot < m;
+    const uint32_t t = tperm ? tperm[tv ? slot : t0] : (tv ? slot : t0);
+    const float4 a = epos[t];
     const int gi = (int)g;
     const size_t G = (size_t)g * g * g;
 
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
         const int zhi = __builtin_amdgcn_readfirstlane(near_hi_cell(mxz, rb, invh, gi));
         const uint32_t nsrc = off[G];
         float4 bpos = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (SEGMENT) bpos = eoth[tv ? t : t0];
+        if (SEGMENT) bpos = eoth[t];
         const int ny_ = yhi - ylo + 1;
         const int nrows = (xlo <= xhi && ylo <= yhi && zlo <= zhi) ? (zhi - zlo + 1) * ny_ : 0;
         for (int r0 = 0; r0 < nrows; r0 += 64) {
@@ -261,11 +264,54 @@ static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st)
     if (a.u_dtype == PPPM_F64)
         hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
                            a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy,
-                           (double*)a.uz, a.u_stride);
+                           (double*)a.uz, a.u_stride, a.tperm);
     else
         hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, float>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
                            a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (float*)a.ux, (float*)a.uy,
-                           (float*)a.uz, a.u_stride);
+                           (float*)a.uz, a.u_stride, a.tperm);
+}
+
+// Evaluation order within each hash cell: by octant of the cell (the halves
+// of x, y and z), original order within an octant, so the points of a
+// cluster share a half-cell and their union box is smaller.  ks: sorted cell
+// keys, pos: sorted origin-shifted points.  A thread ranks its point among
+// its cell's segment; segments longer than kOctMax keep their order (the
+// ranking is quadratic in the segment length).
+constexpr uint32_t kOctMax = 256;
+
+__device__ inline uint32_t octant_of(float4 p, float invh) {
+    const float fx = p.x * invh, fy = p.y * invh, fz = p.z * invh;
+    return (uint32_t)(fx - floorf(fx) >= 0.5f) | ((uint32_t)(fy - floorf(fy) >= 0.5f) << 1) |
+           ((uint32_t)(fz - floorf(fz) >= 0.5f) << 2);
+}
+
+__global__ void __launch_bounds__(256) k_octant_order(const uint32_t* __restrict__ ks, const float4* __restrict__ pos,
+                                                      uint32_t n, float invh, uint32_t* __restrict__ perm) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = ks[p];
+    uint32_t s = p, e = p + 1;
+    while (s > 0 && ks[s - 1] == k && p - s < kOctMax) s--;
+    while (e < n && ks[e] == k && e - p <= kOctMax) e++;
+    if (e - s > kOctMax || (s > 0 && ks[s - 1] == k)) {
+        perm[p] = p;  // long segment: original order
+        return;
+    }
+    const uint32_t o = octant_of(pos[p], invh);
+    uint32_t r = 0;
+    for (uint32_t q = s; q < e; q++) {
+        const uint32_t oq = octant_of(pos[q], invh);
+        r += (oq < o || (oq == o && q < p)) ? 1u : 0u;
+    }
+    perm[s + r] = p;
+}
+
+pppm_status launch_octant_order(const uint32_t* ks, const float4* pos, uint32_t n, double invh, uint32_t* perm,
+                                hipStream_t st) {
+    if (n == 0) return PPPM_OK;
+    hipLaunchKernelGGL(k_octant_order, dim3(div_up(n, 256)), dim3(256), 0, st, ks, pos, n, (float)invh, perm);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
 }
 
 // largest float f with (double)f < x (x > 0; returns -1 otherwise)

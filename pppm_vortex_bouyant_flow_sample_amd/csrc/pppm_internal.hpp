@@ -92,7 +92,11 @@ struct NearArgs {
     void* uz;
     uint32_t u_stride;
     int u_dtype;
+    const uint32_t* tperm = nullptr;  // evaluation slot -> sorted point, or null
 };
+// Octant order of sorted points within their hash cells (see nearfield.hip).
+pppm_status launch_octant_order(const uint32_t* ks, const float4* pos, uint32_t n, double invh, uint32_t* perm,
+                                hipStream_t st);
 pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st);
 pppm_status launch_count_pairs(const NearArgs& a, unsigned long long* d_total, hipStream_t st);
 
