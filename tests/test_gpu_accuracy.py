@@ -35,3 +35,29 @@ def test_plume_8M_256_error_vs_fp64_oracle(native, oracle):
     assert errs["mixed"] <= TOL
     assert errs["fp32"] <= TOL
     assert errs["mixed"] < errs["fp32"]
+
+
+def test_uniform_64k_32_error_vs_direct_sum(native, oracle):
+    """Method error, stated: at the 65,536-vortex uniform scene on 32^3 (K = 2,
+    core 1.5 h, V(2,2) to 1e-6) both GPU paths against the FP64 direct
+    O(N^2) Biot-Savart sum with the same core.  The GPU adds nothing to the
+    method's own error: its distance to the direct sum equals the FP64
+    oracle PPPM's to 1e-4 (relative), and it is within 1e-5 of that PPPM."""
+    n, g = 65536, 32
+    pos, w = scenes.uniform(n)[:2]
+    op = oracle.make_params(g, 1.0 / g, [0, 0, 0], K=2, tol=1e-12, max_cycles=80)
+    print("[direct sum] FP64 direct sum over 65,536^2 pairs ...", flush=True)
+    ud = oracle.direct_sum(pos, w, core_a=op.core_a)
+    uo, _ = oracle.pppm_eval(op, pos, w)
+    e_oracle = rel_l2(uo, ud)
+    print(f"[direct sum] FP64 PPPM vs direct sum: rel L2 = {e_oracle:.4e}", flush=True)
+    scene = native.Scene(pos, w)
+    for name, prec, tol in [("fp32", native.PREC_FP32, 1e-6), ("mixed", native.PREC_MIXED, 1e-7)]:
+        ctx = native.Context(native.config_params(g, precision=prec, mg_tol=tol, mg_max_cycles=40))
+        u = native.solve_self(ctx, scene)
+        e_direct, e_pppm = rel_l2(u, ud), rel_l2(u, uo)
+        print(f"[direct sum] GPU {name}: rel L2 vs direct sum = {e_direct:.4e}, vs FP64 PPPM = {e_pppm:.2e}",
+              flush=True)
+        assert abs(e_direct - e_oracle) <= 1e-4 * e_oracle + 1e-5
+        assert e_pppm <= TOL
+        ctx.close()
