@@ -259,12 +259,13 @@ pppm_status pppm_mg_solve(pppm_mg *mg, void *x, const void *b, int32_t schedule,
 /* iters red-black SOR iterations (odd colour first, MultiGrid_kernel.cu:722-820)
  * on the finest level, in place on x.  blocked (PPPM_F32 only): 0 one launch
  * per half-sweep; 1 temporally blocked LDS passes; 2 colour-split blocked
- * passes (four-plane ring); 3 the same with a six-plane ring.  Kinds 2 and 3
- * take kind 1's kernel when nx is odd. */
+ * passes (four-plane ring); 3 the same with a six-plane ring; 4 kind 3 with
+ * two iterations per pass while two remain.  Kinds 2 to 4 take kind 1's
+ * kernel when nx is odd. */
 pppm_status pppm_mg_smooth(pppm_mg *mg, void *x, const void *b, int32_t iters, double omega, int32_t blocked,
                            void *hip_stream);
 /* PPPM_F32 V-cycles on levels at or above the blocked threshold (below):
- * blocked = 1..3 (the kinds of pppm_mg_smooth; default 2) use fused blocked
+ * blocked = 1..4 (the kinds of pppm_mg_smooth; default 2) use fused blocked
  * passes, 0 one launch per half-sweep, residual, restriction and
  * prolongation. */
 pppm_status pppm_mg_set_blocked(pppm_mg *mg, int32_t blocked);

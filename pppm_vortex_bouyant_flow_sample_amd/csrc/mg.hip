@@ -346,9 +346,11 @@ __global__ void __launch_bounds__(256) k_rb2(TbArgs a) {
     }
 }
 
-// Colour-split blocked smoother: the pass of k_rb2 (one red-black iteration
-// per pass, z planes streamed through LDS rings, halos recomputed, out of
-// place) with each LDS plane split by colour.  Row ly of colour c holds at
+// Colour-split blocked smoother: the pass of k_rb2 (z planes streamed through
+// LDS rings, halos recomputed, out of place) with each LDS plane split by
+// colour, and S half-sweeps per pass (S = 2: one red-black iteration; S = 4:
+// two, which halves the passes over the grid of a V(2,2) cycle's finest
+// levels).  Row ly of colour c holds at
 // index m the colour-c cell of the cell pair (2m, 2m + 1) of that row.  Every
 // neighbour of a colour-c cell is of colour 1 - c: at index m in the rows
 // above and below and in the planes before and after, and at m + s - 1 and
@@ -371,10 +373,12 @@ __global__ void __launch_bounds__(256) k_rb2(TbArgs a) {
 constexpr int kRcTX = 64;
 constexpr int kRcTY = 16;
 
-template <bool RESTRICT, bool PROLONG, bool XZERO, bool BETA, int NR>
+template <bool RESTRICT, bool PROLONG, bool XZERO, bool BETA, int NR, int S>
 __global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
-    constexpr int HXC = RESTRICT ? 4 : 2;        // x halo in cells (even)
-    constexpr int HY = RESTRICT ? 3 : 2;         // y and z halo
+    static_assert(S == 2 || S == 4, "one or two red-black iterations per pass");
+    constexpr int H = S + (RESTRICT ? 1 : 0);    // halo: one cell per half-sweep, +1 for the residual
+    constexpr int HXC = H + (H & 1);             // x halo in cells (even)
+    constexpr int HY = H;                        // y and z halo
     constexpr int LX = kRcTX + 2 * HXC, LY = kRcTY + 2 * HY;
     constexpr int HX = LX / 2, HXP = HX;         // pairs per row (no padding: a wave's reads stay in one bank set)
     constexpr int CP = (LY + 2) * HXP;           // one colour of one plane: rows -1 .. LY
@@ -385,9 +389,9 @@ __global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
     constexpr int NC = (CLX * CLY + 255) / 256;
     constexpr int RW = kRcTX / 2;                // restriction: coarse cells per tile row
     // the next step's store reuses a ring slot this step reads, or the
-    // residual reads what the second half-sweep wrote: barrier after it
-    constexpr bool kBarAfter = RESTRICT || NR < 5;
-    static_assert(NR % 2 == 0 && NR >= (RESTRICT ? 6 : 4), "ring too short");
+    // residual reads what the last half-sweep wrote: barrier after it
+    constexpr bool kBarAfter = RESTRICT || NR < S + 3;
+    static_assert(NR % 2 == 0 && NR >= S + (RESTRICT ? 3 : 2), "ring too short");
     static_assert(256 * (NP - 1) <= NSLOT, "only the last slot may be partial");
     __shared__ float xs[NR * PS];
     __shared__ float rs[RESTRICT ? kRcTY * RW : 1];
@@ -407,7 +411,7 @@ __global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
     const float* xcoarse = PROLONG ? a.xc + (size_t)comp * a.Gc : nullptr;
     const int cx0 = (gx0 >> 1) - 1, cy0 = (gy0 >> 1) - 1;  // coarse window origin
     const int klo = max(z0 - HY, 0), khi = min(z1 + HY, nz);
-    const int kend = khi + 2 + (RESTRICT ? 1 : 0);
+    const int kend = khi + S + (RESTRICT ? 1 : 0);
     const size_t plane = (size_t)nx * ny;
     const float omega = a.omega, beta = a.beta, bscale = a.bscale;
     const int tid = threadIdx.x;
@@ -568,7 +572,7 @@ __global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
                 if (PROLONG && k + 1 < khi && (j & 1) == 1) fetch_coarse(((k + 1) >> 1) + 1);
             }
 #pragma unroll
-            for (int t = 1; t <= 2; t++) {
+            for (int t = 1; t <= S; t++) {
                 const int p = k - t;
                 const int plo = klo > 0 ? klo + t : 0;
                 const int phi = khi < nz ? khi - 1 - t : nz - 1;
@@ -611,13 +615,13 @@ __global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
                         PC[o] = xo + wu[u] * (r * d - xo);
                     }
                 }
-                if (t == 1 || kBarAfter) __syncthreads();
+                if (t < S || kBarAfter) __syncthreads();
             }
-            // plane k - 2 is final: write this thread's interior pairs
+            // plane k - S is final: write this thread's interior pairs
             {
-                const int pw = k - 2;
+                const int pw = k - S;
                 if (pw >= z0 && pw < z1) {
-                    const int jw = (j + 2 * NR - 2) % NR;
+                    const int jw = (j + 2 * NR - S) % NR;
                     const float* P = xs + jw * PS;
                     char* dst = reinterpret_cast<char*>(xdst + (size_t)__builtin_amdgcn_readfirstlane(pw) * plane);
 #pragma unroll
@@ -631,9 +635,9 @@ __global__ void __launch_bounds__(256) k_rbc(TbArgs a) {
                 // residual of plane pr = k - 3 (its neighbours are final) at this
                 // thread's interior pairs, summed over the pair and the plane
                 // pair, then over the two rows of a coarse cell through rs
-                const int pr = k - 3;
+                const int pr = k - S - 1;
                 if (pr >= z0 && pr < z1) {
-                    const int jr = (j + 2 * NR - 3) % NR, jm = (jr + NR - 1) % NR, jq = (jr + 1) % NR;
+                    const int jr = (j + 2 * NR - S - 1) % NR, jm = (jr + NR - 1) % NR, jq = (jr + 1) % NR;
 #pragma unroll
                     for (int u = 0; u < NP; u++) {
                         if (!io[u]) continue;
@@ -834,17 +838,30 @@ __global__ void __launch_bounds__(256) k_resnorm(const T* __restrict__ x, const 
     const T* xc = x + (size_t)comp * G;
     const T* bc = b + (size_t)comp * G;
     double rr = 0, bb = 0;
-    // block b takes a contiguous run of x rows; lanes run along x
+    // block b takes a contiguous run of x rows, RU rows per step so that their
+    // loads are in flight together (rows past the run read its last row and
+    // are not summed); lanes run along x
+    constexpr int RU = 4;
     const int rows = ny * nz, per = (rows + gridDim.x - 1) / gridDim.x;
     const int r0 = blockIdx.x * per, r1 = min(r0 + per, rows);
-    for (int row = r0; row < r1; row++) {
-        const int j = row % ny, k = row / ny;
+    for (int row = r0; row < r1; row += RU) {
         for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-            const size_t t = (size_t)row * nx + i;
-            T r = residual_at(xc, bc, bscale, i, j, k, nx, ny, nz, (T)0);
-            double bv = (double)bscale * (double)bc[t];
-            rr += (double)r * (double)r;
-            bb += bv * bv;
+            T r[RU], bv[RU];
+#pragma unroll
+            for (int u = 0; u < RU; u++) {
+                const int rw = min(row + u, r1 - 1);
+                const int j = rw % ny, k = rw / ny;
+                r[u] = residual_at(xc, bc, bscale, i, j, k, nx, ny, nz, (T)0);
+                bv[u] = bc[(size_t)rw * nx + i];
+            }
+#pragma unroll
+            for (int u = 0; u < RU; u++) {
+                if (row + u < r1) {
+                    const double bd = (double)bscale * (double)bv[u];
+                    rr += (double)r[u] * (double)r[u];
+                    bb += bd * bd;
+                }
+            }
         }
     }
     double s1 = block_sum_256(rr, lds);
@@ -1205,11 +1222,11 @@ static int cu_count() {
 // Launch one k_rbc instantiation with z chunks sized so the whole grid is
 // resident at once (one round of workgroups: no tail), when the tiles leave
 // room for at least two chunks; otherwise the caller's chunking.
-template <bool R, bool P, bool Z, bool B, int NR>
+template <bool R, bool P, bool Z, bool B, int NR, int S>
 static int rbc_per_cu() {
     static int per_cu = [] {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&k_rbc<R, P, Z, B, NR>),
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&k_rbc<R, P, Z, B, NR, S>),
                                                          256, 0) != hipSuccess ||
             n < 1)
             n = 1;
@@ -1222,27 +1239,33 @@ static int rbc_per_cu() {
 // stream capture.
 static void rbc_prime() {
     (void)cu_count();
-    (void)rbc_per_cu<false, false, false, false, 4>();
-    (void)rbc_per_cu<false, false, false, true, 4>();
-    (void)rbc_per_cu<false, false, true, false, 4>();
-    (void)rbc_per_cu<false, false, true, true, 4>();
-    (void)rbc_per_cu<false, true, false, false, 4>();
-    (void)rbc_per_cu<false, true, false, true, 4>();
-    (void)rbc_per_cu<false, false, false, false, 6>();
-    (void)rbc_per_cu<false, false, false, true, 6>();
-    (void)rbc_per_cu<false, false, true, false, 6>();
-    (void)rbc_per_cu<false, false, true, true, 6>();
-    (void)rbc_per_cu<false, true, false, false, 6>();
-    (void)rbc_per_cu<false, true, false, true, 6>();
-    (void)rbc_per_cu<true, false, false, false, 6>();
-    (void)rbc_per_cu<true, false, false, true, 6>();
-    (void)rbc_per_cu<true, false, true, false, 6>();
-    (void)rbc_per_cu<true, false, true, true, 6>();
+    (void)rbc_per_cu<false, false, false, false, 4, 2>();
+    (void)rbc_per_cu<false, false, false, true, 4, 2>();
+    (void)rbc_per_cu<false, false, true, false, 4, 2>();
+    (void)rbc_per_cu<false, false, true, true, 4, 2>();
+    (void)rbc_per_cu<false, true, false, false, 4, 2>();
+    (void)rbc_per_cu<false, true, false, true, 4, 2>();
+    (void)rbc_per_cu<false, false, false, false, 6, 2>();
+    (void)rbc_per_cu<false, false, false, true, 6, 2>();
+    (void)rbc_per_cu<false, false, true, false, 6, 2>();
+    (void)rbc_per_cu<false, false, true, true, 6, 2>();
+    (void)rbc_per_cu<false, true, false, false, 6, 2>();
+    (void)rbc_per_cu<false, true, false, true, 6, 2>();
+    (void)rbc_per_cu<true, false, false, false, 6, 2>();
+    (void)rbc_per_cu<true, false, false, true, 6, 2>();
+    (void)rbc_per_cu<true, false, true, false, 6, 2>();
+    (void)rbc_per_cu<true, false, true, true, 6, 2>();
+    (void)rbc_per_cu<false, true, false, false, 6, 4>();
+    (void)rbc_per_cu<false, true, false, true, 6, 4>();
+    (void)rbc_per_cu<true, false, false, false, 8, 4>();
+    (void)rbc_per_cu<true, false, false, true, 8, 4>();
+    (void)rbc_per_cu<true, false, true, false, 8, 4>();
+    (void)rbc_per_cu<true, false, true, true, 8, 4>();
 }
 
-template <bool R, bool P, bool Z, bool B, int NR>
+template <bool R, bool P, bool Z, bool B, int NR, int S>
 static void launch_rbc(TbArgs a, int ncomp, hipStream_t st) {
-    const int per_cu = rbc_per_cu<R, P, Z, B, NR>();
+    const int per_cu = rbc_per_cu<R, P, Z, B, NR, S>();
     const int tiles = (int)(div_up(a.nx, kRcTX) * div_up(a.ny, kRcTY)) * ncomp;
     const int chunks = per_cu * cu_count() / tiles;
     if (chunks >= 2) {
@@ -1252,26 +1275,33 @@ static void launch_rbc(TbArgs a, int ncomp, hipStream_t st) {
     }
     const int nzc = (a.nz + a.zc - 1) / a.zc;
     dim3 grid(div_up(a.nx, kRcTX), div_up(a.ny, kRcTY), nzc * ncomp), blk(256);
-    hipLaunchKernelGGL((k_rbc<R, P, Z, B, NR>), grid, blk, 0, st, a);
+    hipLaunchKernelGGL((k_rbc<R, P, Z, B, NR, S>), grid, blk, 0, st, a);
 }
 
 // One blocked pass over all components: kind 1 = k_rb2, 2 = k_rbc with a
 // four-plane ring, 3 = k_rbc with a six-plane ring (the residual pass always
 // uses six); k_rbc needs nx even, k_rb2 takes any size.
+// two_iter: two red-black iterations in the pass (k_rbc with S = 4; nx even,
+// kind 4 only), else one.
 template <bool R, bool P, bool Z>
-static pppm_status launch_rb2(const TbArgs& a, int ncomp, int kind, hipStream_t st) {
+static pppm_status launch_rb2(const TbArgs& a, int ncomp, int kind, hipStream_t st, bool two_iter = false) {
     static_assert(kTbTX == kRcTX && kTbTY == kRcTY, "both blocked kernels use the same tiles");
     const int nzc = (a.nz + a.zc - 1) / a.zc;
     dim3 grid(div_up(a.nx, kTbTX), div_up(a.ny, kTbTY), nzc * ncomp), blk(256);
     const bool beta = a.beta != 0.f;
-    if (kind >= 2 && (a.nx & 1) == 0) {
-        if (R || kind == 3) {
-            if (beta) launch_rbc<R, P, Z, true, 6>(a, ncomp, st);
-            else launch_rbc<R, P, Z, false, 6>(a, ncomp, st);
+    if (two_iter && (kind < 4 || (a.nx & 1) != 0)) return PPPM_ERR_INVALID;
+    if (two_iter) {
+        constexpr int NRS4 = R ? 8 : 6;
+        if (beta) launch_rbc<R, P, Z, true, NRS4, 4>(a, ncomp, st);
+        else launch_rbc<R, P, Z, false, NRS4, 4>(a, ncomp, st);
+    } else if (kind >= 2 && (a.nx & 1) == 0) {
+        if (R || kind >= 3) {
+            if (beta) launch_rbc<R, P, Z, true, 6, 2>(a, ncomp, st);
+            else launch_rbc<R, P, Z, false, 6, 2>(a, ncomp, st);
         } else {
             constexpr int NR4 = R ? 6 : 4;
-            if (beta) launch_rbc<R, P, Z, true, NR4>(a, ncomp, st);
-            else launch_rbc<R, P, Z, false, NR4>(a, ncomp, st);
+            if (beta) launch_rbc<R, P, Z, true, NR4, 2>(a, ncomp, st);
+            else launch_rbc<R, P, Z, false, NR4, 2>(a, ncomp, st);
         }
     } else if (beta) {
         hipLaunchKernelGGL((k_rb2<R, P, Z, true>), grid, blk, 0, st, a);
@@ -1283,13 +1313,13 @@ static pppm_status launch_rb2(const TbArgs& a, int ncomp, int kind, hipStream_t 
 }
 
 template <typename T>
-static pppm_status smooth_tb(const T*, T*, const T*, T, int, int, int, int, size_t, T, T, int, hipStream_t) {
+static pppm_status smooth_tb(const T*, T*, const T*, T, int, int, int, int, size_t, T, T, int, bool, hipStream_t) {
     return PPPM_ERR_INVALID;
 }
-// one red-black iteration, xin -> x
+// one red-black iteration (two with two_iter), xin -> x
 template <>
 pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float bscale, int nx, int ny, int nz,
-                             int ncomp, size_t G, float omega, float beta, int kind, hipStream_t st) {
+                             int ncomp, size_t G, float omega, float beta, int kind, bool two_iter, hipStream_t st) {
     TbArgs a{};
     a.xin = xin;
     a.x = x;
@@ -1302,7 +1332,7 @@ pppm_status smooth_tb<float>(const float* xin, float* x, const float* b, float b
     a.omega = omega;
     a.beta = beta;
     a.zc = tb_zchunk(nx, ny, nz, ncomp);
-    return launch_rb2<false, false, false>(a, ncomp, kind, st);
+    return launch_rb2<false, false, false>(a, ncomp, kind, st, two_iter);
 }
 
 template <typename T>
@@ -1321,16 +1351,21 @@ template <typename T>
 pppm_status MGHier<T>::smooth(int l, T* x, const T* b, T bscale, int iters, T omega, T beta, hipStream_t st) {
     const int nx = d_[l][0], ny = d_[l][1], nz = d_[l][2];
     if (std::is_same<T, float>::value && tb_ && iters > 0) {
-        // one out-of-place pass per iteration, x -> t -> x ...; an odd count
-        // ends in t and is copied back
+        // out-of-place passes of one iteration (two under kind 4 while two
+        // remain), x -> t -> x ...; an odd pass count ends in t, copied back
         T* tmp = t_[l];
-        for (int q = 0; q < iters; q++) {
+        const bool pairs = tb_ == 4 && !(nx & 1);
+        int q = 0;
+        for (int done = 0; done < iters; q++) {
+            const bool two = pairs && iters - done >= 2;
             const T* src = (q & 1) ? tmp : x;
             T* dst = (q & 1) ? x : tmp;
-            pppm_status s = smooth_tb<T>(src, dst, b, bscale, nx, ny, nz, ncomp_, cells(l), omega, beta, tb_, st);
+            pppm_status s =
+                smooth_tb<T>(src, dst, b, bscale, nx, ny, nz, ncomp_, cells(l), omega, beta, tb_, two, st);
             if (s != PPPM_OK) return s;
+            done += two ? 2 : 1;
         }
-        if (iters & 1) return copy(x, tmp, cells(l) * ncomp_, st);
+        if (q & 1) return copy(x, tmp, cells(l) * ncomp_, st);
         return PPPM_OK;
     }
     dim3 blk(64, 4), grid(div_up(div_up(nx, 2), 64), div_up(ny, 4), nz * ncomp_);
@@ -1452,14 +1487,17 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
         }
         float* src = X[l];
         float* dst = m.temp(l);
-        for (int q = 0; q < nu1; q++) {
+        // kind 4 runs nu = 2 as one pass of two iterations
+        const bool two1 = kind == 4 && nu1 == 2;
+        for (int q = 0; q < (two1 ? 1 : nu1); q++) {
             TbArgs a = level_args(l);
             a.xin = src;
             a.x = dst;
             a.bn = B[l + 1];
-            const bool zero = (l > 0 || x0_zero) && q == 0, last = q == nu1 - 1;
+            const bool zero = (l > 0 || x0_zero) && q == 0, last = two1 || q == nu1 - 1;
             if (last)
-                s = zero ? launch_rb2<true, false, true>(a, nc, kind, st) : launch_rb2<true, false, false>(a, nc, kind, st);
+                s = zero ? launch_rb2<true, false, true>(a, nc, kind, st, two1)
+                         : launch_rb2<true, false, false>(a, nc, kind, st, two1);
             else
                 s = zero ? launch_rb2<false, false, true>(a, nc, kind, st)
                          : launch_rb2<false, false, false>(a, nc, kind, st);
@@ -1480,13 +1518,14 @@ pppm_status vcycle_fused<float>(MGHier<float>& m, std::vector<float*>& X, std::v
         }
         float* src = cur[l];
         float* dst = src == X[l] ? m.temp(l) : X[l];
-        for (int q = 0; q < nu2; q++) {
+        const bool two2 = kind == 4 && nu2 == 2;
+        for (int q = 0; q < (two2 ? 1 : nu2); q++) {
             TbArgs a = level_args(l);
             a.xin = src;
             a.x = dst;
             a.xc = X[l + 1];
             a.betac = (float)mg_beta(l + 1);
-            s = q == 0 ? launch_rb2<false, true, false>(a, nc, kind, st)
+            s = q == 0 ? launch_rb2<false, true, false>(a, nc, kind, st, two2)
                        : launch_rb2<false, false, false>(a, nc, kind, st);
             if (s != PPPM_OK) return s;
             std::swap(src, dst);

@@ -68,10 +68,11 @@ class MGHier {
     // partial-sum slots per component (the defect-correction step uses one
     // per workgroup: up to 4096 at 1024^3); the residual norm uses kResBlocks
     static constexpr int kNormBlocks = 4096;
-    static constexpr int kResBlocks = 512;
+    static constexpr int kResBlocks = 2048;
     // float smoothing on large levels: 0 one launch per colour, 1 blocked
     // passes (k_rb2), 2 colour-split blocked passes with a four-plane ring,
-    // 3 the same with a six-plane ring
+    // 3 the same with a six-plane ring, 4 kind 3 with two iterations per pass
+    // wherever two follow each other (V-cycles with nu1 or nu2 = 2)
     void set_blocked_smoother(int kind) { tb_ = kind; }
     int blocked_smoother() const { return tb_; }
     // V-cycles run the blocked passes on levels with at least this many cells

@@ -19,16 +19,19 @@ struct NearConst {
 // q(rho) / (4 pi r^3) for the expcubic core, q = 1 - exp(-(r/a)^3)
 // (GPU_kernel.cu:2389-2402), with a series for x = (r/a)^3 < 0.1 to avoid
 // cancellation.  1 / (4 pi) is folded into q (q' = q / (4 pi)), so the
-// coefficient is rinv^3 q'.
+// coefficient is rinv^3 q'.  Pairs that close are rare: every lane takes the
+// exponential form, and the series runs only in waves where some lane needs
+// it (a wave-uniform branch, so other waves issue none of its instructions).
 __device__ inline float coeff_expcubic(float r2, const NearConst& nc) {
     const float rinv = __frsqrt_rn(r2);
     const float r3 = r2 * (r2 * rinv);
-    float q;
-    if (r3 < nc.r3_series) {
+    float q = fmaf(-nc.inv4pi, __builtin_amdgcn_exp2f(nc.kexp2 * r3), nc.inv4pi);
+    const bool small = r3 < nc.r3_series;
+    if (__ballot(small) != 0) {
+        __asm__ volatile("");  // not speculated: keeps the branch a branch
         const float x = r3 * nc.inv_a3;
-        q = x * (nc.sc[0] + x * (nc.sc[1] + x * (nc.sc[2] + x * (nc.sc[3] + x * nc.sc[4]))));
-    } else {
-        q = fmaf(-nc.inv4pi, __builtin_amdgcn_exp2f(nc.kexp2 * r3), nc.inv4pi);
+        const float qs = x * (nc.sc[0] + x * (nc.sc[1] + x * (nc.sc[2] + x * (nc.sc[3] + x * nc.sc[4]))));
+        if (small) q = qs;
     }
     return q * (rinv * rinv * rinv);
 }

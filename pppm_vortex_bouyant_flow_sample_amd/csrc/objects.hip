@@ -295,7 +295,7 @@ extern "C" pppm_status pppm_mg_solve(pppm_mg* m, void* x, const void* b, int32_t
 extern "C" pppm_status pppm_mg_smooth(pppm_mg* m, void* x, const void* b, int32_t iters, double omega,
                                       int32_t blocked, void* stream) {
     if (!m || !x || !b || iters < 0) FAIL(PPPM_ERR_INVALID, "bad argument");
-    if (blocked < 0 || blocked > 3) FAIL(PPPM_ERR_INVALID, "blocked: 0..3");
+    if (blocked < 0 || blocked > 4) FAIL(PPPM_ERR_INVALID, "blocked: 0..4");
     hipStream_t st = (hipStream_t)stream;
     if (m->dtype == PPPM_F32) {
         int prev = m->f.blocked_smoother();
@@ -316,7 +316,7 @@ extern "C" pppm_status pppm_mg_set_blocked_min_cells(pppm_mg* m, uint64_t cells)
 
 extern "C" pppm_status pppm_mg_set_blocked(pppm_mg* m, int32_t blocked) {
     if (!m) FAIL(PPPM_ERR_INVALID, "null mg");
-    if (blocked < 0 || blocked > 3) FAIL(PPPM_ERR_INVALID, "blocked: 0..3");
+    if (blocked < 0 || blocked > 4) FAIL(PPPM_ERR_INVALID, "blocked: 0..4");
     m->f.set_blocked_smoother(blocked);
     return PPPM_OK;
 }

@@ -60,25 +60,26 @@ def test_mg_fp32_reaches_working_precision(native, oracle):
     assert rel_l2(x, xs) < 1e-4
 
 
-@pytest.mark.parametrize("dims,iters", [((64, 64, 64), 2), ((48, 40, 72), 1), ((256, 32, 20), 3), ((9, 7, 5), 2)])
+@pytest.mark.parametrize("dims,iters", [((64, 64, 64), 2), ((48, 40, 72), 1), ((256, 32, 20), 3), ((9, 7, 5), 2),
+                                        ((66, 34, 130), 4)])
 def test_blocked_smoother_equals_halfsweep_kernel(native, dims, iters):
-    """The temporally blocked LDS smoothers (kind 1, and the colour-split
-    kinds 2 and 3, which take kind 1's kernel for odd nx) perform exactly the
-    red-black half-sweeps of the one-launch-per-colour kernel (FP32, rounding
-    only)."""
+    """The temporally blocked LDS smoothers (kind 1, the colour-split kinds 2
+    and 3, and kind 4 with two iterations per pass, which take kind 1's kernel
+    for odd nx) perform exactly the red-black half-sweeps of the
+    one-launch-per-colour kernel (FP32, rounding only)."""
     n = int(np.prod(dims))
     rng = np.random.default_rng(sum(dims))
     x0 = rng.standard_normal(n).astype(np.float32)
     b = rng.standard_normal(n).astype(np.float32)
     mg = native.MG(dims, native.F32)
     out = []
-    for blocked in (0, 1, 2, 3):
+    for blocked in (0, 1, 2, 3, 4):
         print(f"[mg] smooth dims={dims} iters={iters} kind={blocked}", flush=True)
         bx, bb = native.DeviceBuffer.from_array(x0), native.DeviceBuffer.from_array(b)
         native.check(native.lib().pppm_mg_smooth(mg.h, bx.ptr, bb.ptr, iters, native.C.c_double(1.0), blocked, None))
         native.synchronize()
         out.append(bx.to_array(np.float32, n))
-    for q in range(1, 4):
+    for q in range(1, 5):
         assert np.max(np.abs(out[0] - out[q])) <= 1e-5 * np.max(np.abs(out[0])), q
 
 
@@ -91,7 +92,7 @@ def test_fused_vcycle_equals_unfused(native, oracle, dims, nu):
     xs, b = _manufactured(oracle, dims, 9)
     b = b.astype(np.float32)
     res = []
-    for blocked in (0, 1, 2, 3):
+    for blocked in (0, 1, 2, 3, 4):
         print(f"[mg] vcycle dims={dims} nu={nu} kind={blocked}", flush=True)
         mg = native.MG(dims, native.F32)
         mg.set_blocked(blocked)
@@ -99,11 +100,11 @@ def test_fused_vcycle_equals_unfused(native, oracle, dims, nu):
         bx.fill_zero()
         mg.solve(bx.ptr, bb.ptr, native.MG_VCYCLE, nu[0], nu[1], 1.0, 0.0, 2)
         res.append(bx.to_array(np.float32, b.size))
-    for q in range(1, 4):
+    for q in range(1, 5):
         assert rel_l2(res[q], res[0]) < 1e-5, q
 
 
-@pytest.mark.parametrize("kind", [0, 2])
+@pytest.mark.parametrize("kind", [0, 2, 4])
 def test_batched_components_equal_single(native, oracle, kind):
     """Three components solved together (one launch per pass over all of
     them) give each component's single solve, to rounding."""
