@@ -683,6 +683,7 @@ static pppm_status near_common(pppm_ctx* c, uint32_t m, const float4* epos, cons
     a.eidx = eidx;
     a.m = m;
     a.off = c->coff;
+    a.nsrc = c->n;
     a.spos = c->spos;
     a.somg = c->somg;
     a.far = c->far;
@@ -752,6 +753,7 @@ extern "C" pppm_status pppm_count_near_pairs(pppm_ctx* c, uint64_t* pairs) {
     a.epos = c->spos;
     a.m = c->n;
     a.off = c->coff;
+    a.nsrc = c->n;
     a.spos = c->spos;
     a.g = c->g;
     a.K = c->p.K;

@@ -77,7 +77,10 @@ __device__ inline int near_hi_cell(float v, double rb, double invh, int gi) {
 constexpr int kNearTI = 2;
 constexpr int kNearSJ = 64 / kNearTI;
 
-template <int KERNEL, bool SEGMENT, typename TO>
+// WIDE: 64-bit vortex addressing; otherwise (fewer than 2^28 vortices) a
+// vortex's byte offset fits 32 bits and the loads take a scalar base plus a
+// 32-bit lane offset.
+template <int KERNEL, bool SEGMENT, typename TO, bool WIDE>
 __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epos, const float4* __restrict__ eoth,
                                                   const uint32_t* __restrict__ eidx, uint32_t m,
                                                   const uint32_t* __restrict__ off, const float4* __restrict__ spos,
@@ -187,31 +190,33 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
             };
             int rc = 0;
             // The next chunk is loaded into registers while this one is
-            // tested.  Slots past the end of the stream hold a position far
-            // outside every box, so the box test rejects them unasked.
-            const float4 far_away = make_float4(3.0e30f, 3.0e30f, 3.0e30f, 0.f);
-            float4 pp = far_away, po = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (total > 0) {
-                const uint32_t vi = locate(0u, rc);
-                if ((uint32_t)lane < total) {
-                    pp = spos[vi];
-                    po = somg[vi];
+            // tested.  Slots past the end of the stream (pin false) read
+            // vortex 0 and are staged with an x coordinate far outside every
+            // box, so the box test rejects them unasked.
+            float4 pp = make_float4(0.f, 0.f, 0.f, 0.f), po = pp;
+            bool pin = false;
+            auto load = [&](uint32_t S0) {
+                const uint32_t vi = locate(S0, rc);
+                pin = S0 + (uint32_t)lane < total;
+                const uint32_t v = pin ? vi : 0u;
+                if (WIDE) {
+                    pp = spos[v];
+                    po = somg[v];
+                } else {
+                    const uint32_t ob = v << 4;
+                    pp = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(spos) + ob);
+                    po = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(somg) + ob);
                 }
-            }
+            };
+            if (total > 0) load(0u);
             for (uint32_t S0 = 0; S0 < total; S0 += 64) {
+                if (!pin) pp.x = 3.0e30f;
                 cpos[w][lane] = pp;
                 comg[w][lane] = po;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                pp = far_away;
-                if (S0 + 64 < total) {
-                    const uint32_t vi = locate(S0 + 64, rc);
-                    if (S0 + 64 + (uint32_t)lane < total) {
-                        pp = spos[vi];
-                        po = somg[vi];
-                    }
-                }
+                if (S0 + 64 < total) load(S0 + 64);
 #pragma unroll
                 for (int qq = 0; qq < kNearTI; qq++) {
                     const uint32_t q = sj + qq * kNearSJ;
@@ -260,18 +265,23 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
     }
 }
 
-template <int KERNEL, bool SEGMENT>
-static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st) {
+template <int KERNEL, bool SEGMENT, bool WIDE>
+static void launch_sized(const NearArgs& a, const NearConst& nc, hipStream_t st) {
     const uint32_t waves = div_up(a.m, kNearTI);
     dim3 grid(div_up(waves, 4)), blk(256);
     if (a.u_dtype == PPPM_F64)
-        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
-                           a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy,
+        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double, WIDE>), grid, blk, 0, st, a.epos, a.eoth, a.eidx,
+                           a.m, a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy,
                            (double*)a.uz, a.u_stride, a.tperm);
     else
-        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, float>), grid, blk, 0, st, a.epos, a.eoth, a.eidx, a.m,
-                           a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (float*)a.ux, (float*)a.uy,
+        hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, float, WIDE>), grid, blk, 0, st, a.epos, a.eoth, a.eidx,
+                           a.m, a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (float*)a.ux, (float*)a.uy,
                            (float*)a.uz, a.u_stride, a.tperm);
+}
+template <int KERNEL, bool SEGMENT>
+static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st) {
+    if (a.nsrc >= (1u << 28)) launch_sized<KERNEL, SEGMENT, true>(a, nc, st);
+    else launch_sized<KERNEL, SEGMENT, false>(a, nc, st);
 }
 
 // Evaluation order within each hash cell: by octant of the cell (the halves

@@ -79,6 +79,7 @@ struct NearArgs {
     const uint32_t* eidx;    // eval sorted -> caller index
     uint32_t m;
     const uint32_t* off;     // vortex cell offsets [G+1]
+    uint32_t nsrc;           // vortex count (off[G])
     const float4* spos;      // sorted shifted vortex coords
     const float4* somg;      // sorted vortex strengths
     const float* far;        // [3][G]
