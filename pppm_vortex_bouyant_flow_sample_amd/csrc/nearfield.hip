@@ -1,5 +1,5 @@
-// M2P interpolation of the far-field buffer fused with the near-field
-// Biot-Savart correction and the unsort to the caller's order
+// M2P interpolation of the far-field buffer and the near-field Biot-Savart
+// correction added onto it, both written in the caller's order
 // (GPU_kernel.cu:2886-2964 BiotSavartInterpolate_far_field_kernel2,
 //  2262-2436 BiotSavartPPCorrScaleMN_kernel, BiotSavartSolverImp.cpp:452-484).
 #include "pppm_internal.hpp"
@@ -56,11 +56,61 @@ __device__ inline float coeff_gaussian(float r2, const NearConst& nc) {
 // within rb of v.  A vortex's hash cell equals floor of its coordinate over h
 // up to FP32 rounding of the hash (far below 1e-3 cells), so the margin keeps
 // the range a superset of the box; the box test decides.
+// The clamp runs in FP64 before the conversion (lo to [0, 2^31 - 1], hi to
+// [-1, gi - 1]), which gives the saturating conversion's cell for every
+// non-NaN input; a NaN bound yields lo = 0 or hi = -1, an empty or rejected
+// range either way.
 __device__ inline int near_lo_cell(float v, double rb, double invh) {
-    return max(d2i_sat(floor(((double)v - rb) * invh - 1e-3)), 0);
+    return (int)fmin(fmax(floor(((double)v - rb) * invh - 1e-3), 0.0), 2147483647.0);
 }
 __device__ inline int near_hi_cell(float v, double rb, double invh, int gi) {
-    return min(d2i_sat(floor(((double)v + rb) * invh + 1e-3)), gi - 1);
+    return (int)fmin(fmax(floor(((double)v + rb) * invh + 1e-3), -1.0), (double)(gi - 1));
+}
+
+// M2P (GPU_kernel.cu:2886-2964): the far-field buffer interpolated at
+// each sorted evaluation point with hat weights about cell centres (in-grid
+// cells only, corners summed in x-fastest order), into mp[t] for the
+// near-field kernel to add its sum to, or (mp null: no near field) straight
+// to the point's output slot.
+template <typename TO>
+__global__ void __launch_bounds__(256) k_m2p_points(const float4* __restrict__ epos, const uint32_t* __restrict__ eidx,
+                                                    uint32_t m, const float* __restrict__ far, uint32_t g, double invh,
+                                                    float4* __restrict__ mp, TO* __restrict__ ux, TO* __restrict__ uy,
+                                                    TO* __restrict__ uz, uint32_t u_stride) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const float4 a = epos[t];
+    const int gi = (int)g;
+    const size_t G = (size_t)g * g * g;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    const int i = cell_of(a.x, invh), j = cell_of(a.y, invh), k = cell_of(a.z, invh);
+    if (i >= 0 && i < gi && j >= 0 && j < gi && k >= 0 && k < gi) {
+        const double xh = (double)a.x * invh - 0.5, yh = (double)a.y * invh - 0.5, zh = (double)a.z * invh - 0.5;
+        const int i0 = (int)floor(xh), j0 = (int)floor(yh), k0 = (int)floor(zh);
+        const float fx = (float)(xh - i0), fy = (float)(yh - j0), fz = (float)(zh - k0);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int a2 = c & 1, b2 = (c >> 1) & 1, c2 = c >> 2;
+            const int ii = i0 + a2, jj = j0 + b2, kk = k0 + c2;
+            if (ii >= 0 && ii < gi && jj >= 0 && jj < gi && kk >= 0 && kk < gi) {
+                const size_t q = ((size_t)kk * g + jj) * g + ii;
+                const float wgt = (a2 ? fx : 1.f - fx) * (b2 ? fy : 1.f - fy) * (c2 ? fz : 1.f - fz);
+                sx += wgt * far[q];
+                sy += wgt * far[G + q];
+                sz += wgt * far[2 * G + q];
+            }
+        }
+    }
+    if (mp) {
+        mp[t] = make_float4(sx, sy, sz, 0.f);
+        return;
+    }
+    const uint32_t dt = eidx[t];
+    if (dt >= m) return;
+    const size_t dst = (size_t)dt * u_stride;
+    ux[dst] = (TO)sx;
+    uy[dst] = (TO)sy;
+    uz[dst] = (TO)sz;
 }
 
 // Cluster form: one wave takes kNearTI consecutive sorted evaluation points
@@ -71,9 +121,9 @@ __device__ inline int near_hi_cell(float v, double rb, double invh, int gi) {
 // row per lane) and packed into one stream, which is staged 64 vortices at a
 // time into LDS with the next chunk's loads in flight; each lane tests its
 // slice of the chunk against its point and evaluates the kernel where the box
-// test passes.  The M2P corners of a point are shared among its lanes;
-// partial sums are reduced over the slices at the end (fixed order,
-// deterministic).  All loop bounds are wave-uniform.
+// test passes.  Partial sums are reduced over the slices at the end (fixed
+// order, deterministic) and added to the point's M2P value from
+// k_m2p_points.  All loop bounds are wave-uniform.
 constexpr int kNearTI = 2;
 constexpr int kNearSJ = 64 / kNearTI;
 
@@ -87,7 +137,7 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                                                   const float4* __restrict__ somg, const float* __restrict__ far,
                                                   uint32_t g, int K, double invh, NearConst nc, TO* __restrict__ ux,
                                                   TO* __restrict__ uy, TO* __restrict__ uz, uint32_t u_stride,
-                                                  const uint32_t* __restrict__ tperm) {
+                                                  const uint32_t* __restrict__ tperm, const float4* __restrict__ mp) {
     __shared__ float4 cpos[4][64], comg[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ti = lane % kNearTI, sj = lane / kNearTI;
@@ -101,29 +151,8 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
     const int gi = (int)g;
     const size_t G = (size_t)g * g * g;
 
-    // ---- M2P: hat weights about cell centres, in-grid cells only; slice sj < 8
-    // of a point reads corner sj of its 2x2x2 stencil
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    {
-        const int i = cell_of(a.x, invh), j = cell_of(a.y, invh), k = cell_of(a.z, invh);
-        if (tv && sj < 8 && i >= 0 && i < gi && j >= 0 && j < gi && k >= 0 && k < gi) {
-            const double xh = (double)a.x * invh - 0.5, yh = (double)a.y * invh - 0.5, zh = (double)a.z * invh - 0.5;
-            const int i0 = (int)floor(xh), j0 = (int)floor(yh), k0 = (int)floor(zh);
-            const float fx = (float)(xh - i0), fy = (float)(yh - j0), fz = (float)(zh - k0);
-            const int a2 = sj & 1, b2 = (sj >> 1) & 1, c2 = sj >> 2;
-            const int ii = i0 + a2, jj = j0 + b2, kk = k0 + c2;
-            if (ii >= 0 && ii < gi && jj >= 0 && jj < gi && kk >= 0 && kk < gi) {
-                const size_t q = ((size_t)kk * g + jj) * g + ii;
-                const float wgt = (a2 ? fx : 1.f - fx) * (b2 ? fy : 1.f - fy) * (c2 ? fz : 1.f - fz);
-                sx = wgt * far[q];
-                sy = wgt * far[G + q];
-                sz = wgt * far[2 * G + q];
-            }
-        }
-    }
-
     float vx = 0.f, vy = 0.f, vz = 0.f;
-    if (K >= 0) {
+    {
         // ---- union of the points' boxes -> scanned cell range (wave-uniform)
         float mnx = tv ? a.x : 3.0e38f, mny = tv ? a.y : 3.0e38f, mnz = tv ? a.z : 3.0e38f;
         float mxx = tv ? a.x : -3.0e38f, mxy = tv ? a.y : -3.0e38f, mxz = tv ? a.z : -3.0e38f;
@@ -248,19 +277,21 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
         }
     }
     // ---- reduce over the slices of each point
-    float rx = sx + vx, ry = sy + vy, rz = sz + vz;
+    float rx = vx, ry = vy, rz = vz;
     for (int o = kNearTI; o < 64; o <<= 1) {
         rx += __shfl_xor(rx, o, 64);
         ry += __shfl_xor(ry, o, 64);
         rz += __shfl_xor(rz, o, 64);
     }
+    // plus the point's M2P value
     if (sj == 0 && tv) {
         const uint32_t dt = eidx[t];
         if (dt < m) {
+            const float4 f = mp[t];
             const size_t dst = (size_t)dt * u_stride;
-            ux[dst] = (TO)rx;
-            uy[dst] = (TO)ry;
-            uz[dst] = (TO)rz;
+            ux[dst] = (TO)(f.x + rx);
+            uy[dst] = (TO)(f.y + ry);
+            uz[dst] = (TO)(f.z + rz);
         }
     }
 }
@@ -272,11 +303,11 @@ static void launch_sized(const NearArgs& a, const NearConst& nc, hipStream_t st)
     if (a.u_dtype == PPPM_F64)
         hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, double, WIDE>), grid, blk, 0, st, a.epos, a.eoth, a.eidx,
                            a.m, a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (double*)a.ux, (double*)a.uy,
-                           (double*)a.uz, a.u_stride, a.tperm);
+                           (double*)a.uz, a.u_stride, a.tperm, a.m2p);
     else
         hipLaunchKernelGGL((k_m2p_near<KERNEL, SEGMENT, float, WIDE>), grid, blk, 0, st, a.epos, a.eoth, a.eidx,
                            a.m, a.off, a.spos, a.somg, a.far, a.g, a.K, a.invh, nc, (float*)a.ux, (float*)a.uy,
-                           (float*)a.uz, a.u_stride, a.tperm);
+                           (float*)a.uz, a.u_stride, a.tperm, a.m2p);
 }
 template <int KERNEL, bool SEGMENT>
 static void launch_typed(const NearArgs& a, const NearConst& nc, hipStream_t st) {
@@ -351,6 +382,20 @@ pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st) {
         for (int k = 0; k < 5; k++) nc.sc[k] = (float)(c[k] / kRefFourPi);
     }
     (void)h;
+    if (a.K >= 0 && !a.m2p) return PPPM_ERR_INVALID;
+    {
+        // with no near field (vortex-in-cell, K < 0) M2P writes the outputs
+        float4* mp = a.K >= 0 ? a.m2p : nullptr;
+        dim3 grid(div_up(a.m, 256)), blk(256);
+        if (a.u_dtype == PPPM_F64)
+            hipLaunchKernelGGL(k_m2p_points<double>, grid, blk, 0, st, a.epos, a.eidx, a.m, a.far, a.g, a.invh, mp,
+                               (double*)a.ux, (double*)a.uy, (double*)a.uz, a.u_stride);
+        else
+            hipLaunchKernelGGL(k_m2p_points<float>, grid, blk, 0, st, a.epos, a.eidx, a.m, a.far, a.g, a.invh, mp,
+                               (float*)a.ux, (float*)a.uy, (float*)a.uz, a.u_stride);
+        PPPM_LAUNCH_CHECK();
+    }
+    if (a.K < 0) return PPPM_OK;
     const bool seg = a.eoth != nullptr;
     if (a.kernel == PPPM_KERNEL_GAUSSIAN) {
         if (seg) launch_typed<PPPM_KERNEL_GAUSSIAN, true>(a, nc, st);

@@ -94,6 +94,7 @@ struct NearArgs {
     uint32_t u_stride;
     int u_dtype;
     const uint32_t* tperm = nullptr;  // evaluation slot -> sorted point, or null
+    float4* m2p = nullptr;            // [m] M2P values in sorted order (scratch)
 };
 // Octant order of sorted points within their hash cells (see nearfield.hip).
 pppm_status launch_octant_order(const uint32_t* ks, const float4* pos, uint32_t n, double invh, uint32_t* perm,
