@@ -214,6 +214,10 @@ struct pppm_ctx {
     hipGraphExec_t g_plain = nullptr;
     hipGraphExec_t g_warm = nullptr;
 
+    // Sort keys carry each point's octant in its cell (8 G keys fit 32 bits):
+    // points come out of the sort in the near field's evaluation order.
+    bool octant_keys() const { return G * 8 <= ((size_t)1 << 32); }
+
     void drop_graphs() {
         for (hipGraphExec_t* e : {&g_mixed[0], &g_mixed[1], &g_plain, &g_warm})
             if (*e) {
@@ -520,14 +524,15 @@ extern "C" pppm_status pppm_set_vortices(pppm_ctx* c, uint32_t n, const float* p
     TRY(c->ensure_vort(n));
     c->n = n;
     c->have_far = false;
-    TRY(launch_calc_keys((const float4*)pos4, n, c->pg.hash, c->vs.k0, nullptr, st));
-    TRY(radix_sort_pairs(c->vs.k0, c->vs.v0, c->vs.k1, c->vs.v1, n, key_bits_for(c->G), true, c->vs.scratch, &c->vs.ks,
-                         &c->vs.vs, st));
-    TRY(launch_cell_tables(c->vs.ks, n, (uint32_t)c->G, c->cstart, c->cend, st));
+    const uint32_t osh = c->octant_keys() ? 3u : 0u;
+    TRY(launch_calc_keys((const float4*)pos4, n, c->pg.hash, c->vs.k0, nullptr, st, osh != 0));
+    TRY(radix_sort_pairs(c->vs.k0, c->vs.v0, c->vs.k1, c->vs.v1, n, key_bits_for(c->G) + osh, true, c->vs.scratch,
+                         &c->vs.ks, &c->vs.vs, st));
+    TRY(launch_cell_tables(c->vs.ks, n, (uint32_t)c->G, c->cstart, c->cend, st, osh));
     TRY(cell_offsets_from_tables(c->cstart, c->cend, (uint32_t)c->G, c->coff, c->scan_scr, st));
     TRY(launch_fill_u32(c->d_strays, 1, 0u, st));
     TRY(launch_reorder_vortices(c->vs.vs, c->vs.ks, (const float4*)pos4, wx, wy, wz, w_stride, w_dtype, n, c->pg,
-                                c->spos, c->somg, c->d_strays, st));
+                                c->spos, c->somg, c->d_strays, st, osh));
     c->have_vort = true;
     c->stats.n = n;
     return PPPM_OK;
@@ -722,9 +727,13 @@ extern "C" pppm_status pppm_evaluate_self(pppm_ctx* c, void* ux, void* uy, void*
     if (u_dtype != PPPM_F32 && u_dtype != PPPM_F64) FAIL(PPPM_ERR_INVALID, "u_dtype");
     if (u_stride == 0) FAIL(PPPM_ERR_INVALID, "u_stride");
     c->stats.m = c->n;
-    // octant order within cells, into the sort's spare index buffer
-    uint32_t* perm = c->vs.vs == c->vs.v0 ? c->vs.v1 : c->vs.v0;
-    TRY(launch_octant_order(c->vs.ks, c->spos, c->n, c->pg.invh, perm, c->st));
+    // octant order within cells: from the sort when the keys carry the
+    // octant, else ranked into the sort's spare index buffer
+    uint32_t* perm = nullptr;
+    if (!c->octant_keys()) {
+        perm = c->vs.vs == c->vs.v0 ? c->vs.v1 : c->vs.v0;
+        TRY(launch_octant_order(c->vs.ks, c->spos, c->n, c->pg.invh, perm, c->st));
+    }
     return near_common(c, c->n, c->spos, nullptr, c->vs.vs, perm, ux, uy, uz, u_stride, u_dtype);
 }
 
@@ -738,14 +747,18 @@ extern "C" pppm_status pppm_evaluate(pppm_ctx* c, uint32_t m, const float* eval_
     if (m > (1u << 30)) FAIL(PPPM_ERR_INVALID, "too many points");
     hipStream_t st = c->st;
     TRY(c->ensure_eval(m, other_end4 != nullptr));
-    TRY(launch_calc_keys((const float4*)eval_pos4, m, c->pg.hash, c->es.k0, nullptr, st));
-    TRY(radix_sort_pairs(c->es.k0, c->es.v0, c->es.k1, c->es.v1, m, key_bits_for(c->G), true, c->es.scratch,
-                         &c->es.ks, &c->es.vs, st));
+    const bool ok8 = c->octant_keys();
+    TRY(launch_calc_keys((const float4*)eval_pos4, m, c->pg.hash, c->es.k0, nullptr, st, ok8));
+    TRY(radix_sort_pairs(c->es.k0, c->es.v0, c->es.k1, c->es.v1, m, key_bits_for(c->G) + (ok8 ? 3u : 0u), true,
+                         c->es.scratch, &c->es.ks, &c->es.vs, st));
     TRY(launch_reorder_eval(c->es.vs, (const float4*)eval_pos4, (const float4*)other_end4, m, c->pg, c->epos,
                             other_end4 ? c->eoth : nullptr, st));
     c->stats.m = m;
-    uint32_t* perm = c->es.vs == c->es.v0 ? c->es.v1 : c->es.v0;
-    TRY(launch_octant_order(c->es.ks, c->epos, m, c->pg.invh, perm, st));
+    uint32_t* perm = nullptr;
+    if (!ok8) {
+        perm = c->es.vs == c->es.v0 ? c->es.v1 : c->es.v0;
+        TRY(launch_octant_order(c->es.ks, c->epos, m, c->pg.invh, perm, st));
+    }
     return near_common(c, m, c->epos, other_end4 ? c->eoth : nullptr, c->es.vs, perm, ux, uy, uz, u_stride,
                        u_dtype);
 }

@@ -16,19 +16,21 @@ constexpr int kRadix = 1 << kRadixBits;
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_calc_keys(const float4* __restrict__ pos, uint32_t n, uint32_t g, float hf,
                                                    float ox, float oy, float oz, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ idx) {
+                                                   uint32_t* __restrict__ idx, bool octant) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 p = pos[i];
-    keys[i] = hash_cell_key(p.x, p.y, p.z, g, hf, ox, oy, oz);
+    keys[i] = octant ? hash_cell_octant_key(p.x, p.y, p.z, g, hf, ox, oy, oz)
+                     : hash_cell_key(p.x, p.y, p.z, g, hf, ox, oy, oz);
     if (idx) idx[i] = i;
 }
 
 pppm_status launch_calc_keys(const float4* pos, uint32_t n, const HashGeom& hg, uint32_t* keys, uint32_t* idx,
-                             hipStream_t st) {
+                             hipStream_t st, bool octant) {
     if (n == 0) return PPPM_OK;
+    if (octant && (uint64_t)hg.g * hg.g * hg.g * 8 > ((uint64_t)1 << 32)) return PPPM_ERR_INVALID;
     hipLaunchKernelGGL(k_calc_keys, dim3(div_up(n, 256)), dim3(256), 0, st, pos, n, hg.g, hg.hf, hg.of[0], hg.of[1],
-                       hg.of[2], keys, idx);
+                       hg.of[2], keys, idx, octant);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }
@@ -227,14 +229,17 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, uint64_t n, uint32_t v) {
 }
 
 // particle_hash_kernel.cu:74-126 (FindCellStartD), neighbour key read directly.
+// cell = key >> shift
 __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ ks, uint32_t n, uint32_t ncells,
-                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                     uint32_t shift) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t k = ks[i];
-    if (i == 0 || k != ks[i - 1]) {
+    uint32_t k = ks[i] >> shift;
+    const uint32_t kp = i > 0 ? ks[i - 1] >> shift : 0u;
+    if (i == 0 || k != kp) {
         if (k < ncells) start[k] = i;
-        if (i > 0 && ks[i - 1] < ncells) end[ks[i - 1]] = i;
+        if (i > 0 && kp < ncells) end[kp] = i;
     }
     if (i == n - 1 && k < ncells) end[k] = n;
 }
@@ -249,13 +254,14 @@ pppm_status launch_fill_u32(uint32_t* a, uint64_t n, uint32_t v, hipStream_t st)
 }
 
 pppm_status launch_cell_tables(const uint32_t* keys_sorted, uint32_t n, uint32_t ncells, uint32_t* start,
-                               uint32_t* end, hipStream_t st) {
+                               uint32_t* end, hipStream_t st, uint32_t shift) {
     pppm_status s = launch_fill_u32(start, ncells, kEmptyCell, st);
     if (s != PPPM_OK) return s;
     s = launch_fill_u32(end, ncells, kEmptyCell, st);
     if (s != PPPM_OK) return s;
     if (n == 0) return PPPM_OK;
-    hipLaunchKernelGGL(k_cell_bounds, dim3(div_up(n, 256)), dim3(256), 0, st, keys_sorted, n, ncells, start, end);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(div_up(n, 256)), dim3(256), 0, st, keys_sorted, n, ncells, start, end,
+                       shift);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }
@@ -324,7 +330,7 @@ __global__ void __launch_bounds__(256) k_reorder_vortices(const uint32_t* __rest
                                                           uint32_t w_stride, uint32_t n, uint32_t g, double ox,
                                                           double oy, double oz, double invh,
                                                           float4* __restrict__ spos, float4* __restrict__ somg,
-                                                          uint32_t* __restrict__ strays) {
+                                                          uint32_t* __restrict__ strays, uint32_t key_shift) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool stray = false;
     if (i < n && idx[i] < n) {
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(256) k_reorder_vortices(const uint32_t* __rest
         uint32_t dkey = kEmptyCell;
         if (ix >= 0 && ix < (int)g && iy >= 0 && iy < (int)g && iz >= 0 && iz < (int)g)
             dkey = ((uint32_t)iz * g + (uint32_t)iy) * g + (uint32_t)ix;
-        stray = (dkey != kEmptyCell) && (dkey != keys_sorted[i]);
+        stray = (dkey != kEmptyCell) && (dkey != (keys_sorted[i] >> key_shift));
         spos[i] = make_float4(qx, qy, qz, __uint_as_float(dkey));
         somg[i] = make_float4((float)wx[j * w_stride], (float)wy[j * w_stride], (float)wz[j * w_stride], 0.0f);
     }
@@ -346,17 +352,17 @@ __global__ void __launch_bounds__(256) k_reorder_vortices(const uint32_t* __rest
 pppm_status launch_reorder_vortices(const uint32_t* idx, const uint32_t* keys_sorted, const float4* pos,
                                     const void* wx, const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
                                     uint32_t n, const PathGeom& pg, float4* spos, float4* somg, uint32_t* strays,
-                                    hipStream_t st) {
+                                    hipStream_t st, uint32_t key_shift) {
     if (n == 0) return PPPM_OK;
     dim3 grid(div_up(n, 256)), blk(256);
     if (w_dtype == PPPM_F64)
         hipLaunchKernelGGL(k_reorder_vortices<double>, grid, blk, 0, st, idx, keys_sorted, pos, (const double*)wx,
                            (const double*)wy, (const double*)wz, w_stride, n, pg.hash.g, pg.origin[0], pg.origin[1],
-                           pg.origin[2], pg.invh, spos, somg, strays);
+                           pg.origin[2], pg.invh, spos, somg, strays, key_shift);
     else if (w_dtype == PPPM_F32)
         hipLaunchKernelGGL(k_reorder_vortices<float>, grid, blk, 0, st, idx, keys_sorted, pos, (const float*)wx,
                            (const float*)wy, (const float*)wz, w_stride, n, pg.hash.g, pg.origin[0], pg.origin[1],
-                           pg.origin[2], pg.invh, spos, somg, strays);
+                           pg.origin[2], pg.invh, spos, somg, strays, key_shift);
     else
         return PPPM_ERR_INVALID;
     PPPM_LAUNCH_CHECK();

@@ -66,6 +66,20 @@ __device__ inline uint32_t hash_cell_key(float px, float py, float pz, uint32_t 
     uint32_t uz = (uint32_t)f2i_sat(floorf(qz)) % g;
     return (uz * g + uy) * g + ux;
 }
+// 8 * hash_cell_key + the point's octant in its cell: bit 0 (1, 2) set when
+// the fractional part of the x (y, z) quotient is at least 1/2.
+__device__ inline uint32_t hash_cell_octant_key(float px, float py, float pz, uint32_t g, float hf, float ox,
+                                                float oy, float oz) {
+    float qx = __fdiv_rn(__fsub_rn(px, ox), hf);
+    float qy = __fdiv_rn(__fsub_rn(py, oy), hf);
+    float qz = __fdiv_rn(__fsub_rn(pz, oz), hf);
+    float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+    uint32_t ux = (uint32_t)f2i_sat(fx) % g;
+    uint32_t uy = (uint32_t)f2i_sat(fy) % g;
+    uint32_t uz = (uint32_t)f2i_sat(fz) % g;
+    uint32_t o = (uint32_t)(qx - fx >= 0.5f) | ((uint32_t)(qy - fy >= 0.5f) << 1) | ((uint32_t)(qz - fz >= 0.5f) << 2);
+    return ((uz * g + uy) * g + ux) * 8u + o;
+}
 
 // Position relative to the double origin, rounded to float
 // (GPU_kernel.cu:252-255: float4 ppos; ppos.x -= origin.x).

@@ -27,8 +27,11 @@ struct HashGeom {
 };
 size_t sort_scratch_bytes(uint32_t n);
 uint32_t key_bits_for(uint64_t ncells);
+// octant: key = 8 * cell + octant of the point within its cell (the
+// halves in x, y and z of the hash quotient; needs 8 * cells <= 2^32), so a
+// sort orders each cell's points by octant.
 pppm_status launch_calc_keys(const float4* pos, uint32_t n, const HashGeom& hg, uint32_t* keys, uint32_t* idx,
-                             hipStream_t st);
+                             hipStream_t st, bool octant = false);
 // Sorts (keys, vals) by key, stable; on return *keys_out/*vals_out point at
 // whichever buffer pair holds the result.  vals_identity: the input values
 // are 0..n-1 and vals is not read.
@@ -36,8 +39,9 @@ pppm_status radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                              uint32_t key_bits, bool vals_identity, void* scratch, uint32_t** keys_out,
                              uint32_t** vals_out, hipStream_t st);
 pppm_status launch_fill_u32(uint32_t* a, uint64_t n, uint32_t v, hipStream_t st);
+// cell = key >> shift (3 for octant keys)
 pppm_status launch_cell_tables(const uint32_t* keys_sorted, uint32_t n, uint32_t ncells, uint32_t* start,
-                               uint32_t* end, hipStream_t st);
+                               uint32_t* end, hipStream_t st, uint32_t shift = 0);
 pppm_status launch_gather(const uint32_t* idx, const void* in, void* out, uint32_t stride, int dtype, uint32_t n,
                           hipStream_t st);
 pppm_status launch_scatter(const uint32_t* idx, const void* in, void* out, uint32_t stride, int dtype, uint32_t n,
@@ -57,7 +61,7 @@ struct PathGeom {
 pppm_status launch_reorder_vortices(const uint32_t* idx, const uint32_t* keys_sorted, const float4* pos,
                                     const void* wx, const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
                                     uint32_t n, const PathGeom& pg, float4* spos, float4* somg, uint32_t* strays,
-                                    hipStream_t st);
+                                    hipStream_t st, uint32_t key_shift = 0);
 // Sorted evaluation points: shifted coords, optional other end (shifted).
 pppm_status launch_reorder_eval(const uint32_t* idx, const float4* pos, const float4* other, uint32_t m,
                                 const PathGeom& pg, float4* epos, float4* eoth, hipStream_t st);
