@@ -88,16 +88,24 @@ __global__ void __launch_bounds__(256) k_m2p_points(const float4* __restrict__ e
         const double xh = (double)a.x * invh - 0.5, yh = (double)a.y * invh - 0.5, zh = (double)a.z * invh - 0.5;
         const int i0 = (int)floor(xh), j0 = (int)floor(yh), k0 = (int)floor(zh);
         const float fx = (float)(xh - i0), fy = (float)(yh - j0), fz = (float)(zh - k0);
+        // the lower corner is at most one cell outside the grid per axis and
+        // the upper one inside, so per axis: lower valid iff >= 0, upper iff
+        // < gi.  Cell indices fit 32 bits (g <= 1024); a lower corner at -1
+        // wraps and the +1 of the upper corner brings it back.
+        const bool lx = i0 >= 0, ux = i0 + 1 < gi, ly = j0 >= 0, uy = j0 + 1 < gi, lz = k0 >= 0, uz = k0 + 1 < gi;
+        const uint32_t q0 = ((uint32_t)k0 * g + (uint32_t)j0) * g + (uint32_t)i0, gg = g * g;
+        const char* Fx = reinterpret_cast<const char*>(far);
+        const char* Fy = reinterpret_cast<const char*>(far + G);
+        const char* Fz = reinterpret_cast<const char*>(far + 2 * G);
 #pragma unroll
         for (int c = 0; c < 8; c++) {
             const int a2 = c & 1, b2 = (c >> 1) & 1, c2 = c >> 2;
-            const int ii = i0 + a2, jj = j0 + b2, kk = k0 + c2;
-            if (ii >= 0 && ii < gi && jj >= 0 && jj < gi && kk >= 0 && kk < gi) {
-                const size_t q = ((size_t)kk * g + jj) * g + ii;
+            if ((a2 ? ux : lx) && (b2 ? uy : ly) && (c2 ? uz : lz)) {
+                const uint32_t ob = 4u * (q0 + (uint32_t)a2 + (b2 ? g : 0u) + (c2 ? gg : 0u));
                 const float wgt = (a2 ? fx : 1.f - fx) * (b2 ? fy : 1.f - fy) * (c2 ? fz : 1.f - fz);
-                sx += wgt * far[q];
-                sy += wgt * far[G + q];
-                sz += wgt * far[2 * G + q];
+                sx += wgt * *reinterpret_cast<const float*>(Fx + ob);
+                sy += wgt * *reinterpret_cast<const float*>(Fy + ob);
+                sz += wgt * *reinterpret_cast<const float*>(Fz + ob);
             }
         }
     }

@@ -19,7 +19,25 @@ __global__ void __launch_bounds__(256) k_p2m(const uint32_t* __restrict__ off, c
     const uint32_t n = off[G];
     uint32_t s = off[c], e = min(off[c + 1], n);
     Acc ax = 0, ay = 0, az = 0;
-    for (uint32_t p = s; p < e; p++) {
+    // four vortices' loads in flight per step, summed in segment order
+    uint32_t p = s;
+    for (; p + 4 <= e; p += 4) {
+        uint32_t k[4];
+        float4 o[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            k[u] = __float_as_uint(spos[p + u].w);
+            o[u] = somg[p + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k[u] != c) continue;
+            ax += (Acc)o[u].x;
+            ay += (Acc)o[u].y;
+            az += (Acc)o[u].z;
+        }
+    }
+    for (; p < e; p++) {
         if (__float_as_uint(spos[p].w) != c) continue;
         float4 o = somg[p];
         ax += (Acc)o.x;
