@@ -531,8 +531,12 @@ extern "C" pppm_status pppm_set_vortices(pppm_ctx* c, uint32_t n, const float* p
     TRY(launch_cell_tables(c->vs.ks, n, (uint32_t)c->G, c->cstart, c->cend, st, osh));
     TRY(cell_offsets_from_tables(c->cstart, c->cend, (uint32_t)c->G, c->coff, c->scan_scr, st));
     TRY(launch_fill_u32(c->d_strays, 1, 0u, st));
-    TRY(launch_reorder_vortices(c->vs.vs, c->vs.ks, (const float4*)pos4, wx, wy, wz, w_stride, w_dtype, n, c->pg,
-                                c->spos, c->somg, c->d_strays, st, osh));
+    {
+        // the sort's spare value buffer holds the inverse permutation
+        uint32_t* inv = c->vs.vs == c->vs.v0 ? c->vs.v1 : c->vs.v0;
+        TRY(launch_reorder_vortices_scatter(c->vs.vs, inv, (const float4*)pos4, wx, wy, wz, w_stride, w_dtype, n,
+                                            c->pg, c->spos, c->somg, c->d_strays, st));
+    }
     c->have_vort = true;
     c->stats.n = n;
     return PPPM_OK;

@@ -322,52 +322,73 @@ __device__ inline int deposit_index(float q, double invh) {
     return i;
 }
 
+// Sorted vortex arrays (particle_hash_kernel.cu:128-184 applyReorder), by
+// scatter: vortex j (caller order, read contiguously) goes to its sorted slot
+// inv[j] (a gather from the caller's arrays reads whole 128-byte lines per
+// 16- and 4-byte element).  spos = origin-shifted float coordinates with the
+// deposit cell in .w; somg = strengths.  A stray is a vortex whose deposit
+// cell is not its hash cell; the hash cell is recomputed from the position
+// (the cell its sort key holds).
 template <typename W>
-__global__ void __launch_bounds__(256) k_reorder_vortices(const uint32_t* __restrict__ idx,
-                                                          const uint32_t* __restrict__ keys_sorted,
-                                                          const float4* __restrict__ pos, const W* __restrict__ wx,
-                                                          const W* __restrict__ wy, const W* __restrict__ wz,
-                                                          uint32_t w_stride, uint32_t n, uint32_t g, double ox,
-                                                          double oy, double oz, double invh,
-                                                          float4* __restrict__ spos, float4* __restrict__ somg,
-                                                          uint32_t* __restrict__ strays, uint32_t key_shift) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_reorder_vortices_scatter(
+    const uint32_t* __restrict__ inv, const float4* __restrict__ pos, const W* __restrict__ wx,
+    const W* __restrict__ wy, const W* __restrict__ wz, uint32_t w_stride, uint32_t n, uint32_t g, float hf,
+    float hox, float hoy, float hoz, double ox, double oy, double oz, double invh, float4* __restrict__ spos,
+    float4* __restrict__ somg, uint32_t* __restrict__ strays) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     bool stray = false;
-    if (i < n && idx[i] < n) {
-        size_t j = idx[i];
-        float4 p = pos[j];
-        float qx = shift_coord(p.x, ox), qy = shift_coord(p.y, oy), qz = shift_coord(p.z, oz);
-        int ix = deposit_index(qx, invh), iy = deposit_index(qy, invh), iz = deposit_index(qz, invh);
+    if (j < n) {
+        const uint32_t i = inv[j];
+        const float4 p = pos[j];
+        const float qx = shift_coord(p.x, ox), qy = shift_coord(p.y, oy), qz = shift_coord(p.z, oz);
+        const int ix = deposit_index(qx, invh), iy = deposit_index(qy, invh), iz = deposit_index(qz, invh);
         uint32_t dkey = kEmptyCell;
         if (ix >= 0 && ix < (int)g && iy >= 0 && iy < (int)g && iz >= 0 && iz < (int)g)
             dkey = ((uint32_t)iz * g + (uint32_t)iy) * g + (uint32_t)ix;
-        stray = (dkey != kEmptyCell) && (dkey != (keys_sorted[i] >> key_shift));
-        spos[i] = make_float4(qx, qy, qz, __uint_as_float(dkey));
-        somg[i] = make_float4((float)wx[j * w_stride], (float)wy[j * w_stride], (float)wz[j * w_stride], 0.0f);
+        stray = (dkey != kEmptyCell) && (dkey != hash_cell_key(p.x, p.y, p.z, g, hf, hox, hoy, hoz));
+        const size_t js = (size_t)j * w_stride;
+        const float4 w = make_float4((float)wx[js], (float)wy[js], (float)wz[js], 0.0f);
+        if (i < n) {
+            spos[i] = make_float4(qx, qy, qz, __uint_as_float(dkey));
+            somg[i] = w;
+        }
     }
     uint64_t m = __ballot(stray);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(strays, (uint32_t)__popcll(m));
 }
 
-pppm_status launch_reorder_vortices(const uint32_t* idx, const uint32_t* keys_sorted, const float4* pos,
-                                    const void* wx, const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
-                                    uint32_t n, const PathGeom& pg, float4* spos, float4* somg, uint32_t* strays,
-                                    hipStream_t st, uint32_t key_shift) {
+// inv[idx[i]] = i
+__global__ void __launch_bounds__(256) k_invert_perm(const uint32_t* __restrict__ idx, uint32_t n,
+                                                     uint32_t* __restrict__ inv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = idx[i];
+    if (j < n) inv[j] = i;
+}
+
+pppm_status launch_reorder_vortices_scatter(const uint32_t* idx, uint32_t* inv, const float4* pos, const void* wx,
+                                            const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
+                                            uint32_t n, const PathGeom& pg, float4* spos, float4* somg,
+                                            uint32_t* strays, hipStream_t st) {
     if (n == 0) return PPPM_OK;
     dim3 grid(div_up(n, 256)), blk(256);
+    hipLaunchKernelGGL(k_invert_perm, grid, blk, 0, st, idx, n, inv);
+    PPPM_LAUNCH_CHECK();
+    const HashGeom& hg = pg.hash;
     if (w_dtype == PPPM_F64)
-        hipLaunchKernelGGL(k_reorder_vortices<double>, grid, blk, 0, st, idx, keys_sorted, pos, (const double*)wx,
-                           (const double*)wy, (const double*)wz, w_stride, n, pg.hash.g, pg.origin[0], pg.origin[1],
-                           pg.origin[2], pg.invh, spos, somg, strays, key_shift);
+        hipLaunchKernelGGL(k_reorder_vortices_scatter<double>, grid, blk, 0, st, inv, pos, (const double*)wx,
+                           (const double*)wy, (const double*)wz, w_stride, n, hg.g, hg.hf, hg.of[0], hg.of[1],
+                           hg.of[2], pg.origin[0], pg.origin[1], pg.origin[2], pg.invh, spos, somg, strays);
     else if (w_dtype == PPPM_F32)
-        hipLaunchKernelGGL(k_reorder_vortices<float>, grid, blk, 0, st, idx, keys_sorted, pos, (const float*)wx,
-                           (const float*)wy, (const float*)wz, w_stride, n, pg.hash.g, pg.origin[0], pg.origin[1],
-                           pg.origin[2], pg.invh, spos, somg, strays, key_shift);
+        hipLaunchKernelGGL(k_reorder_vortices_scatter<float>, grid, blk, 0, st, inv, pos, (const float*)wx,
+                           (const float*)wy, (const float*)wz, w_stride, n, hg.g, hg.hf, hg.of[0], hg.of[1],
+                           hg.of[2], pg.origin[0], pg.origin[1], pg.origin[2], pg.invh, spos, somg, strays);
     else
         return PPPM_ERR_INVALID;
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }
+
 
 __global__ void __launch_bounds__(256) k_reorder_eval(const uint32_t* __restrict__ idx, const float4* __restrict__ pos,
                                                       const float4* __restrict__ other, uint32_t m, double ox,

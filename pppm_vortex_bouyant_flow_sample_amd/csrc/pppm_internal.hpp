@@ -56,12 +56,14 @@ struct PathGeom {
     double invh;   // 1 / h
 };
 
-// Sorted vortex gather: spos = origin-shifted float coords + deposit-cell key
+// Sorted vortex arrays: spos = origin-shifted float coords + deposit-cell key
 // in .w (as bits; 0xFFFFFFFF outside the grid); somg = strengths (x,y,z,0).
-pppm_status launch_reorder_vortices(const uint32_t* idx, const uint32_t* keys_sorted, const float4* pos,
-                                    const void* wx, const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
-                                    uint32_t n, const PathGeom& pg, float4* spos, float4* somg, uint32_t* strays,
-                                    hipStream_t st, uint32_t key_shift = 0);
+// idx: sorted slot -> caller index; inv (n words of scratch) receives its
+// inverse.
+pppm_status launch_reorder_vortices_scatter(const uint32_t* idx, uint32_t* inv, const float4* pos, const void* wx,
+                                            const void* wy, const void* wz, uint32_t w_stride, int w_dtype,
+                                            uint32_t n, const PathGeom& pg, float4* spos, float4* somg,
+                                            uint32_t* strays, hipStream_t st);
 // Sorted evaluation points: shifted coords, optional other end (shifted).
 pppm_status launch_reorder_eval(const uint32_t* idx, const float4* pos, const float4* other, uint32_t m,
                                 const PathGeom& pg, float4* epos, float4* eoth, hipStream_t st);
