@@ -146,7 +146,8 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
                                                   uint32_t g, int K, double invh, NearConst nc, TO* __restrict__ ux,
                                                   TO* __restrict__ uy, TO* __restrict__ uz, uint32_t u_stride,
                                                   const uint32_t* __restrict__ tperm, const float4* __restrict__ mp) {
-    __shared__ float4 cpos[4][64], comg[4][64];
+    // per wave: a ring of 128 staged vortices (two windows of 64)
+    __shared__ float4 cpos[4][128], comg[4][128];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ti = lane % kNearTI, sj = lane / kNearTI;
     const uint32_t t0 = (blockIdx.x * 4 + w) * kNearTI;
@@ -185,6 +186,39 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
         if (SEGMENT) bpos = eoth[t];
         const int ny_ = yhi - ylo + 1;
         const int nrows = (xlo <= xhi && ylo <= yhi && zlo <= zhi) ? (zhi - zlo + 1) * ny_ : 0;
+        // Evaluate the window [wh, wh + n) of the ring (wh is 0 or 64): lane
+        // (point, slice) tests ring entries wh + slice and wh + slice + 32.
+        auto evaluate = [&](uint32_t wh, uint32_t n) {
+#pragma unroll
+            for (int qq = 0; qq < kNearTI; qq++) {
+                const uint32_t q = sj + qq * kNearSJ;
+                const float4 s = cpos[w][wh + q];
+                const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                // |d|inf <= rbox as one max3 of absolute values; a point
+                // past the end of the evaluation set (tv false) computes on
+                // a copy of its cluster's first point and is not stored
+                bool acc = q < n && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) <= rbox && r2 >= 1e-24f;
+                if (SEGMENT) {
+                    const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
+                    acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
+                }
+                if (acc) {
+                    const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
+                                                                    : coeff_expcubic(r2, nc);
+                    const float4 o = comg[w][wh + q];
+                    vx += (o.y * dz - o.z * dy) * cf;
+                    vy += (o.z * dx - o.x * dz) * cf;
+                    vz += (o.x * dy - o.y * dx) * cf;
+                }
+            }
+            // the next staging may overwrite these entries
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        };
+        // ring state (wave-uniform): window head (0 or 64) and entries held
+        uint32_t qh = 0, qn = 0;
         for (int r0 = 0; r0 < nrows; r0 += 64) {
             // lane l holds the vortex range [lp0, lp1) of row r0 + l (z outer,
             // y inner), loaded in one pass instead of row after row
@@ -227,9 +261,8 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
             };
             int rc = 0;
             // The next chunk is loaded into registers while this one is
-            // tested.  Slots past the end of the stream (pin false) read
-            // vortex 0 and are staged with an x coordinate far outside every
-            // box, so the box test rejects them unasked.
+            // staged.  Slots past the end of the stream (pin false) read
+            // vortex 0 and are not staged.
             float4 pp = make_float4(0.f, 0.f, 0.f, 0.f), po = pp;
             bool pin = false;
             auto load = [&](uint32_t S0) {
@@ -247,42 +280,36 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
             };
             if (total > 0) load(0u);
             for (uint32_t S0 = 0; S0 < total; S0 += 64) {
-                if (!pin) pp.x = 3.0e30f;
-                cpos[w][lane] = pp;
-                comg[w][lane] = po;
+                // Stage only the vortices some point's box may accept: per
+                // axis max(min - s, s - max) over the points' coordinates.
+                // Each difference rounds no higher than the box test's
+                // difference to the nearer point (rounding is monotone), so
+                // the filter keeps every vortex the box test accepts.
+                const float fx = fmaxf(mnx - pp.x, pp.x - mxx), fy = fmaxf(mny - pp.y, pp.y - mxy);
+                const float fz = fmaxf(mnz - pp.z, pp.z - mxz);
+                const bool use = pin && fmaxf(fmaxf(fx, fy), fz) <= rbox;
+                const uint64_t bal = __ballot(use);
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (use) {
+                    const uint32_t e = (qh + qn + below) & 127u;
+                    cpos[w][e] = pp;
+                    comg[w][e] = po;
+                }
+                qn += (uint32_t)__popcll(bal);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 if (S0 + 64 < total) load(S0 + 64);
-#pragma unroll
-                for (int qq = 0; qq < kNearTI; qq++) {
-                    const uint32_t q = sj + qq * kNearSJ;
-                    const float4 s = cpos[w][q];
-                    const float dx = a.x - s.x, dy = a.y - s.y, dz = a.z - s.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    // |d|inf <= rbox as one max3 of absolute values; a point
-                    // past the end of the evaluation set (tv false) computes
-                    // on a copy of its cluster's first point and is not stored
-                    bool acc = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) <= rbox && r2 >= 1e-24f;
-                    if (SEGMENT) {
-                        const float ex = bpos.x - s.x, ey = bpos.y - s.y, ez = bpos.z - s.z;
-                        acc = acc && ex * ex + ey * ey + ez * ez >= 1e-24f;
-                    }
-                    if (acc) {
-                        const float cf = KERNEL == PPPM_KERNEL_GAUSSIAN ? coeff_gaussian(r2, nc)
-                                                                        : coeff_expcubic(r2, nc);
-                        const float4 o = comg[w][q];
-                        vx += (o.y * dz - o.z * dy) * cf;
-                        vy += (o.z * dx - o.x * dz) * cf;
-                        vz += (o.x * dy - o.y * dx) * cf;
-                    }
+                // a full window: evaluate it and move the head to the other
+                if (qn >= 64) {
+                    evaluate(qh, 64u);
+                    qh ^= 64u;
+                    qn -= 64u;
                 }
-                // the next chunk overwrites the staging slots
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
+        if (qn > 0) evaluate(qh, qn);  // the last, partial window
     }
     // ---- reduce over the slices of each point
     float rx = vx, ry = vy, rz = vz;
