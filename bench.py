@@ -8,10 +8,10 @@ float32 strengths) are resident in HBM before the timed region.
 
 Three paths are timed on the same inputs:
   served : the mixed-precision path behind the request server (continuous
-           batching): `--slots` requests of this workload (default 2) are kept
+           batching): `--slots` requests of this workload (default 3) are kept
            in flight, each on its own resident context and stream, so one
-           request's host-synchronous multigrid loop overlaps the other's
-           device work.  One step = one request; latency is per request,
+           request's host-synchronous multigrid loop and memory-bound stages
+           overlap the others' device work.  One step = one request; latency is per request,
            submit to done.
   mixed  : the same mixed-precision path, one evaluation after another
            (FP32 multigrid inside FP64 defect correction to relative
@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--workload", default="plume-8M-256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="served,mixed,fp32")
-    ap.add_argument("--slots", type=int, default=2)
+    ap.add_argument("--slots", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run an identity gather of 16M float4 rows (4 B index + 16 B read, 16 B written "
                          "per row) three times, the known-size kernel that HBM counters are scaled by")
