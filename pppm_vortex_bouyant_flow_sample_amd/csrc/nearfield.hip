@@ -187,7 +187,9 @@ __global__ void __launch_bounds__(256) k_m2p_near(const float4* __restrict__ epo
         const int ny_ = yhi - ylo + 1;
         const int nrows = (xlo <= xhi && ylo <= yhi && zlo <= zhi) ? (zhi - zlo + 1) * ny_ : 0;
         // Evaluate the window [wh, wh + n) of the ring (wh is 0 or 64): lane
-        // (point, slice) tests ring entries wh + slice and wh + slice + 32.
+        // (point, slice) tests ring entries wh + slice + kNearSJ j, j <
+        // kNearTI.  (A ring of 256 with 128-entry windows halves the
+        // occupancy through its LDS and runs half again as long.)
         auto evaluate = [&](uint32_t wh, uint32_t n) {
 #pragma unroll
             for (int qq = 0; qq < kNearTI; qq++) {
