@@ -118,6 +118,32 @@ def test_eval_points_and_segment_mode(native, oracle):
     assert rel_l2(u1, u2) > 1e-6  # the exclusion changed something
 
 
+@pytest.mark.parametrize("m", [1, 4097, 4098, 4099])
+def test_eval_point_counts_not_multiple_of_cluster(native, oracle, m):
+    """The near field takes 4 sorted points per wave; counts that leave 1-3
+    idle points in the last wave (and a single point) match the oracle, and
+    every slot of the output is written."""
+    pos, w = scenes.plume(20_000, seed=31)[:2]
+    rng = np.random.default_rng(m)
+    ev = np.zeros((m, 4), np.float32)
+    ev[:, :3] = pos[rng.integers(0, pos.shape[0], m), :3] + rng.normal(0, 0.004, (m, 3)).astype(np.float32)
+    p = native.config_params(32, precision=1, mg_tol=1e-9)
+    ctx = native.Context(p)
+    sc = native.Scene(pos, w)
+    ctx.set_vortices(sc.n, sc.pos.ptr, sc.w_ptrs())
+    ctx.compute_far_field()
+    be = native.DeviceBuffer.from_array(ev)
+    out = [native.DeviceBuffer.from_array(np.full(m, np.nan)) for _ in range(3)]
+    ctx.evaluate(m, be.ptr, None, [b.ptr for b in out])
+    native.synchronize()
+    u = np.stack([b.to_array(np.float64, m) for b in out])
+    assert np.all(np.isfinite(u))
+    uo, _ = oracle.pppm_eval(_oracle_params(oracle, p), pos, w, eval_pos4=ev)
+    err = rel_l2(u, uo)
+    print(f"[pipeline] {m} evaluation points rel L2 = {err:.3e}", flush=True)
+    assert err <= TOL_MIXED
+
+
 def test_reference_preset_fmg(native, oracle):
     """The reference's own settings: 3x bounding-box domain, a = 0.08, one FMG
     pass of V(4,4) SOR 1.43f, K = 1 (VortexSimulationSystemImp.cpp:890-905)."""
