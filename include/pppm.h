@@ -284,6 +284,16 @@ pppm_status pppm_memset(void *dst, int value, size_t bytes);
 pppm_status pppm_synchronize(void);
 const char *pppm_last_error(void);
 
+/* ---- frame output (newTest.cpp:23-37 output()) ------------------------------ */
+/* Writes dir/Particle_data%04d.bin: n particles as raw float4 (x, y, z, w),
+ * w = 1 as the reference writes it, or w[i] when w is given (the reference
+ * renderer reads w as the particle's density, read_particles.cpp:55-70).
+ * pos4 (float4 per particle) and w are device pointers; the copy runs on
+ * hip_stream (null: the default stream) in chunks of 2^20 particles through
+ * pinned host memory.  PPPM_ERR_INVALID if the file cannot be written. */
+pppm_status pppm_write_frame(const char *dir, int32_t frame, const float *pos4, const float *w, uint32_t n,
+                             void *hip_stream);
+
 /* ---- timing on the library's streams --------------------------------------- */
 typedef struct pppm_timer pppm_timer;
 pppm_status pppm_timer_create(pppm_timer **out);

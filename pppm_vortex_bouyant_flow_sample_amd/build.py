@@ -27,7 +27,7 @@ FLAGS = [
 
 SOURCES = [
     "scan.hip", "hash.hip", "p2m.hip", "mg.hip", "farfield.hip", "nearfield.hip",
-    "api.hip", "objects.hip", "ref_abi.hip", "serve.hip", "particle_mesh.hip",
+    "api.hip", "objects.hip", "ref_abi.hip", "serve.hip", "particle_mesh.hip", "frames.hip",
 ]
 
 

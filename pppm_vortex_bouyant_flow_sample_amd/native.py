@@ -136,6 +136,21 @@ def dev_to_array(ptr, dtype, count):
     return out
 
 
+def write_frame(directory, frame, pos_ptr, n, w_ptr=None, stream=None):
+    """pppm_write_frame: directory/Particle_data%04d.bin from device float4
+    positions (w = 1, or the device scalars at w_ptr)."""
+    def vp(p):
+        return p if isinstance(p, C.c_void_p) or p is None else C.c_void_p(p)
+
+    check(lib().pppm_write_frame(str(directory).encode(), C.c_int32(frame), vp(pos_ptr), vp(w_ptr), C.c_uint32(n),
+                                 vp(stream)), "write_frame")
+
+
+def read_frame(path):
+    """A frame file as an (n, 4) float32 array (x, y, z, w)."""
+    return np.fromfile(path, dtype=np.float32).reshape(-1, 4)
+
+
 def config_params(g, **kw):
     p = Params()
     lib().pppm_params_config_preset(C.byref(p), C.c_uint32(g))
