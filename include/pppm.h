@@ -284,6 +284,19 @@ pppm_status pppm_memset(void *dst, int value, size_t bytes);
 pppm_status pppm_synchronize(void);
 const char *pppm_last_error(void);
 
+/* Tracer evaluation as the simulation's caller runs it
+ * (VortexSimulationSystemImp.cpp:1194-1244 evaluateBiotSavart): the m points
+ * in batches of at most `batch`, each batch's velocity (evaluated like
+ * pppm_evaluate, FP64) added into vel4 (double4 per point,
+ * GPU_kernel.cu:3589-3606 AddURotation); when move != 0 each point is then
+ * advanced by forward Euler, pos = (float)((double)pos + vel dt) with w = 0
+ * (GPU_kernel.cu:3118-3140 move_particle).  pos4 and vel4 are device
+ * pointers, updated in place on the context's stream.  The reference's batch
+ * is 163,840 points; any batch gives the same velocities to FP32 rounding.
+ * Requires pppm_compute_far_field. */
+pppm_status pppm_evaluate_tracers(pppm_ctx *ctx, uint32_t m, float *pos4, double *vel4, uint32_t batch, double dt,
+                                  int32_t move);
+
 /* ---- frame output (newTest.cpp:23-37 output()) ------------------------------ */
 /* Writes dir/Particle_data%04d.bin: n particles as raw float4 (x, y, z, w),
  * w = 1 as the reference writes it, or w[i] when w is given (the reference

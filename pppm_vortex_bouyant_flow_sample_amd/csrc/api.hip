@@ -192,6 +192,9 @@ struct pppm_ctx {
     // M2P value of each sorted evaluation point (k_m2p_points -> near field)
     float4* m2p = nullptr;
     uint32_t capP = 0;
+    // tracer batches: FP64 velocities of one batch, [3][capT]
+    double* utr = nullptr;
+    uint32_t capT = 0;
     uint32_t *cstart = nullptr, *cend = nullptr, *coff = nullptr, *scan_scr = nullptr;
     uint32_t* d_strays = nullptr;
     uint32_t* h_strays = nullptr;
@@ -300,6 +303,7 @@ struct pppm_ctx {
         dev_free(spos);
         dev_free(somg);
         dev_free(m2p);
+        dev_free(utr);
         dev_free(cstart);
         dev_free(cend);
         dev_free(coff);
@@ -765,6 +769,51 @@ extern "C" pppm_status pppm_evaluate(pppm_ctx* c, uint32_t m, const float* eval_
     }
     return near_common(c, m, c->epos, other_end4 ? c->eoth : nullptr, c->es.vs, perm, ux, uy, uz, u_stride,
                        u_dtype);
+}
+
+// add_u_rotation_kernel (GPU_kernel.cu:3589-3606): vel.xyz += u; then, when
+// dt is applied, move_particle_kernel (GPU_kernel.cu:3118-3140): the position
+// advanced by vel dt in FP64 and stored as float with w = 0.
+__global__ void __launch_bounds__(256) k_tracer_update(float4* __restrict__ pos, double4* __restrict__ vel,
+                                                       const double* __restrict__ u, uint32_t n, uint32_t cap,
+                                                       double dt, int move) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    v.x += u[i];
+    v.y += u[(size_t)cap + i];
+    v.z += u[2 * (size_t)cap + i];
+    vel[i] = v;
+    if (move) {
+        const float4 p = pos[i];
+        pos[i] = make_float4((float)((double)p.x + v.x * dt), (float)((double)p.y + v.y * dt),
+                             (float)((double)p.z + v.z * dt), 0.0f);
+    }
+}
+
+extern "C" pppm_status pppm_evaluate_tracers(pppm_ctx* c, uint32_t m, float* pos4, double* vel4, uint32_t batch,
+                                             double dt, int32_t move) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (!c->have_far) FAIL(PPPM_ERR_STATE, "compute_far_field first");
+    if (m > 0 && (!pos4 || !vel4)) FAIL(PPPM_ERR_INVALID, "null array");
+    if (batch == 0) FAIL(PPPM_ERR_INVALID, "batch must be > 0");
+    const uint32_t cap = batch < m ? batch : (m > 0 ? m : 1);
+    if (cap > c->capT || !c->utr) {
+        dev_free(c->utr);
+        TRY(dev_alloc(&c->utr, 3 * (size_t)cap));
+        c->capT = cap;
+    }
+    for (uint32_t b0 = 0; b0 < m; b0 += cap) {
+        const uint32_t nb = m - b0 < cap ? m - b0 : cap;
+        double* u = c->utr;
+        TRY(pppm_evaluate(c, nb, pos4 + 4 * (size_t)b0, nullptr, u, u + c->capT, u + 2 * (size_t)c->capT, 1,
+                          PPPM_F64));
+        hipLaunchKernelGGL(k_tracer_update, dim3(div_up(nb, 256)), dim3(256), 0, c->st,
+                           reinterpret_cast<float4*>(pos4) + b0, reinterpret_cast<double4*>(vel4) + b0, u, nb,
+                           c->capT, dt, move ? 1 : 0);
+        PPPM_LAUNCH_CHECK();
+    }
+    return PPPM_OK;
 }
 
 extern "C" pppm_status pppm_solve_self(pppm_ctx* c, uint32_t n, const float* pos4, const void* wx, const void* wy,

@@ -215,6 +215,10 @@ class Context:
         check(lib().pppm_evaluate(self.h, C.c_uint32(m), eval_buf, other_buf, ux, uy, uz, C.c_uint32(u_stride),
                                   C.c_int32(u_dtype)), "evaluate")
 
+    def evaluate_tracers(self, m, pos_buf, vel_buf, batch, dt=0.0, move=False):
+        check(lib().pppm_evaluate_tracers(self.h, C.c_uint32(m), pos_buf, vel_buf, C.c_uint32(batch),
+                                          C.c_double(dt), C.c_int32(1 if move else 0)), "evaluate_tracers")
+
     def stats(self):
         s = Stats()
         check(lib().pppm_get_stats(self.h, C.byref(s)), "get_stats")
