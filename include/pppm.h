@@ -307,6 +307,52 @@ pppm_status pppm_evaluate_tracers(pppm_ctx *ctx, uint32_t m, float *pos4, double
 pppm_status pppm_write_frame(const char *dir, int32_t frame, const float *pos4, const float *w, uint32_t n,
                              void *hip_stream);
 
+/* ---- time step of the vortex-segment method --------------------------------
+ * VortexSimulationSystemImp.cpp:90-125 (solveTimeStep) and its kernels,
+ * GPU_kernel.cu:3118-3254, 3327-3460, 3589-3606.  Device pointers; positions
+ * are float4 (x, y, z, w), velocities double4; enqueued on hip_stream (null:
+ * the default stream).  FP64 arithmetic rounded through float where the
+ * reference's is, with no fused multiply-adds.  n = 0 does nothing. */
+
+/* ComputePosb (GPU_kernel.cu:3182-3235): segment ends enda = c - s/2,
+ * endb = c + s/2 about each centre, s = w when |w| < h (kappa = 1), else
+ * s = w h / |w| (kappa = |w| / h).  The ends' w fields are not written. */
+pppm_status pppm_segments_from_vortices(const float *cent4, float *enda4, float *endb4, const double *wx,
+                                        const double *wy, const double *wz, double *kappa, double h, uint32_t n,
+                                        void *hip_stream);
+/* move_particle (GPU_kernel.cu:3118-3140): pos += u dt in FP64, w = 0. */
+pppm_status pppm_move_points(float *pos4, const double *u4, double dt, uint32_t n, void *hip_stream);
+/* ComputePosa (GPU_kernel.cu:3237-3254): cent.xyz = 0.5 (enda + endb), the
+ * sum in float; cent.w is not written. */
+pppm_status pppm_segment_centres(const float *enda4, const float *endb4, float *cent4, uint32_t n,
+                                 void *hip_stream);
+/* applyFlow without a boundary (VortexSimulationSystemImp.cpp:277-300): both
+ * ends moved by their velocities and the centre re-formed, in one pass; the
+ * same results as two pppm_move_points and one pppm_segment_centres. */
+pppm_status pppm_apply_flow(float *enda4, float *endb4, float *cent4, const double *ua4, const double *ub4,
+                            double dt, uint32_t n, void *hip_stream);
+/* ComputeVortex (GPU_kernel.cu:3143-3180): dw = kappa (endb - enda) - w, the
+ * difference of positions in float. */
+pppm_status pppm_vortices_from_segments(const float *enda4, const float *endb4, const double *wx,
+                                        const double *wy, const double *wz, double *dwx, double *dwy,
+                                        double *dwz, const double *kappa, uint32_t n, void *hip_stream);
+/* VectorAdd (GPU_kernel.cu:3442-3460): C = a A + b B (C may be A or B). */
+pppm_status pppm_vector_add(const double *A, const double *B, double *C, double a, double b, uint32_t n,
+                            void *hip_stream);
+/* AddURotation (GPU_kernel.cu:3589-3606): U.xyz += (u, v, w). */
+pppm_status pppm_add_velocity(double *U4, const double *u, const double *v, const double *w, uint32_t n,
+                              void *hip_stream);
+/* computeStretching's smoothing (VortexSimulationSystemImp.cpp:466-520,
+ * ParticleGaussianSmooth GPU_kernel.cu:3327-3438) over the vortices of the
+ * last pppm_set_vortices: o[i] = sum_p e^{-d^2/(2 nu^2)} v[p] / sum_p
+ * e^{-d^2/(2 nu^2)} over the vortices p of the hash cells within +-2 of i's
+ * cell (the reference passes nu = 0.1).  v and o are caller-order FP64 with
+ * strides (in elements); o may be v.  Runs on the context stream; the
+ * context keeps n double4 of scratch. */
+pppm_status pppm_smooth_vorticity(pppm_ctx *ctx, const double *vx, const double *vy, const double *vz,
+                                  uint32_t v_stride, double *ox, double *oy, double *oz, uint32_t o_stride,
+                                  double nu);
+
 /* ---- timing on the library's streams --------------------------------------- */
 typedef struct pppm_timer pppm_timer;
 pppm_status pppm_timer_create(pppm_timer **out);

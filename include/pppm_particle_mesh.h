@@ -59,6 +59,27 @@ void BiotSavartComputeFarField(uint32_t *start, uint32_t *end, float4 *pos, doub
 void BiotSavartInterpolateFarField(float4 *pos, double *far_u, double *far_v, double *far_w, double *p_u, double *p_v,
                                    double *p_w, double cell_h, uint32_t dimx, uint32_t dimy, uint32_t dimz,
                                    uint32_t num_particles, double4 origin);
+/* Time step (VortexSimulationSystemImp.cpp:90-125), ParticleMesh.h:254-309:
+ * ComputePosb GPU_kernel.cu:3182-3235, move_particle 3118-3140 (w = 0),
+ * ComputePosa 3237-3254, ComputeVortex 3143-3180, VectorAdd 3442-3460,
+ * AddURotation 3589-3606.  Same arithmetic as the pppm_* entry points of the
+ * same names in pppm.h. */
+void ComputePosb(float4 *posa, float4 *posc, float4 *posb, double *vortx, double *vorty, double *vortz, double *k,
+                 double h, uint32_t num_particles);
+void move_particle(float4 *pos, double dt, double4 *U, uint32_t num_particles);
+void ComputePosa(float4 *posc, float4 *posb, float4 *posa, uint32_t num_particles);
+void ComputeVortex(float4 *posa, float4 *posb, double *vortx, double *vorty, double *vortz, double *dvortx,
+                   double *dvorty, double *dvortz, double *k, uint32_t num_particles);
+void VectorAdd(double *A, double *B, double *C, double a, double b, uint32_t num);
+void AddURotation(double4 *U, double *u, double *v, double *w, uint32_t num);
+
+/* ParticleMesh.h:264-279, GPU_kernel.cu:3327-3438: o[t] = Gaussian-weighted
+ * mean (weight e^{-d^2/(2 nu^2)}) of the sorted i over the sorted particles
+ * of the hash cells within +-2 of pos[t]'s cell; dt is unused, as in the
+ * reference. */
+void ParticleGaussianSmooth(uint32_t *start, uint32_t *end, float4 *pos, float4 *pos_sort, double nu, double dt,
+                            double *o_vortx, double *o_vorty, double *o_vortz, double *i_vortx, double *i_vorty,
+                            double *i_vortz, double h, uint3 gdim, uint32_t N, double4 origin);
 }
 
 #endif

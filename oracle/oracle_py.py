@@ -264,3 +264,62 @@ def mg_prolong_tri(xf, xc, fdims):
     xf = _f64(xf).copy()
     lib().orc_mg_prolong_tri(_p(xf), _p(_f64(xc)), *[C.c_int(d) for d in fdims])
     return xf
+
+
+# ---- time step of the vortex-segment method --------------------------------
+def segments_from_vortices(cent4, omega, h):
+    """ComputePosb: (enda4, endb4, kappa).  The ends' w fields are copied from
+    cent4's (the reference leaves them as they were)."""
+    cent4 = _f32(cent4)
+    n = cent4.shape[0]
+    enda, endb = cent4.copy(), cent4.copy()
+    kappa = np.empty(n, np.float64)
+    wx, wy, wz = (_f64(omega[i]) for i in range(3))
+    lib().orc_segments_from_vortices(_p(cent4), _p(enda), _p(endb), _p(wx), _p(wy), _p(wz), _p(kappa),
+                                     C.c_double(h), C.c_uint32(n))
+    return enda, endb, kappa
+
+
+def move_points(pos4, dt, u4):
+    """move_particle: a moved copy of pos4 (w = 0)."""
+    pos4 = _f32(pos4).copy()
+    lib().orc_move_points(_p(pos4), C.c_double(dt), _p(_f64(u4)), C.c_uint32(pos4.shape[0]))
+    return pos4
+
+
+def segment_centres(posc4, posb4, posa4):
+    """ComputePosa: a copy of posa4 with x, y, z the ends' midpoints."""
+    out = _f32(posa4).copy()
+    lib().orc_segment_centres(_p(_f32(posc4)), _p(_f32(posb4)), _p(out), C.c_uint32(out.shape[0]))
+    return out
+
+
+def vortices_from_segments(posa4, posb4, omega, kappa):
+    """ComputeVortex: dw (3, N) = kappa (b - a) - w."""
+    posa4 = _f32(posa4)
+    n = posa4.shape[0]
+    wx, wy, wz = (_f64(omega[i]) for i in range(3))
+    d = [np.empty(n, np.float64) for _ in range(3)]
+    lib().orc_vortices_from_segments(_p(posa4), _p(_f32(posb4)), _p(wx), _p(wy), _p(wz), _p(d[0]), _p(d[1]),
+                                     _p(d[2]), _p(_f64(kappa)), C.c_uint32(n))
+    return np.stack(d)
+
+
+def gaussian_smooth(params, pos4, values, nu=0.1):
+    """computeStretching's smoothing (VortexSimulationSystemImp.cpp:466-520):
+    the particles hashed on the params' grid, the values (3, N) sorted with
+    them, then ParticleGaussianSmooth; returns (3, N) in caller order."""
+    pos4 = _f32(pos4)
+    n = pos4.shape[0]
+    g = params.g
+    key, idx = hash_keys(pos4, g, params.hf, list(params.originf))
+    key, idx = sort_pairs(key, idx)
+    start, end = cell_tables(key, g ** 3)
+    spos = np.ascontiguousarray(pos4[idx])
+    iv = [np.ascontiguousarray(_f64(values[c])[idx]) for c in range(3)]
+    out = [np.empty(n, np.float64) for _ in range(3)]
+    o = (C.c_double * 3)(*[float(params.origin[i]) for i in range(3)])
+    lib().orc_gaussian_smooth(_p(start), _p(end), _p(pos4), _p(spos), C.c_double(nu), _p(out[0]), _p(out[1]),
+                              _p(out[2]), _p(iv[0]), _p(iv[1]), _p(iv[2]), C.c_double(float(np.float32(params.hf))),
+                              C.c_int(g), C.c_uint32(n), o)
+    return np.stack(out)

@@ -1153,3 +1153,124 @@ void orc_direct_sum(uint32_t n, const float *vort_pos4, const double *wx,
         uz[t] = vz;
     }
 }
+
+/* ---- time step of the vortex-segment method ---------------------------
+ * VortexSimulationSystemImp.cpp:90-125 (solveTimeStep) and the kernels it
+ * calls.  Positions are float4 arrays (x, y, z, w); only the fields the
+ * reference writes are written. */
+
+/* GPU_kernel.cu:3182-3235 compute_posb_kernel / ComputePosb: segment ends
+ * enda = c - s/2, endb = c + s/2 with s = w when |w| < h (kappa = 1), else
+ * s = w h / |w| (kappa = |w| / h). */
+void orc_segments_from_vortices(const float *cent4, float *enda4, float *endb4, const double *wx,
+                                const double *wy, const double *wz, double *kappa, double h, uint32_t n)
+{
+    uint32_t i;
+    for (i = 0; i < n; i++) {
+        double s[3] = {wx[i], wy[i], wz[i]};
+        double l = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        int c;
+        kappa[i] = l < h ? 1.0 : l / h;
+        for (c = 0; c < 3; c++) {
+            double half = l < h ? 0.5 * s[c] : 0.5 * s[c] / l * h;
+            double a = (double)cent4[4 * (size_t)i + c];
+            endb4[4 * (size_t)i + c] = (float)(a + half);
+            enda4[4 * (size_t)i + c] = (float)(a - half);
+        }
+    }
+}
+
+/* GPU_kernel.cu:3118-3140 move_particle_kernel: pos += u dt in FP64, stored
+ * as float with w = 0.  u is double4 (x, y, z, w). */
+void orc_move_points(float *pos4, double dt, const double *u4, uint32_t n)
+{
+    uint32_t i;
+    for (i = 0; i < n; i++) {
+        int c;
+        for (c = 0; c < 3; c++)
+            pos4[4 * (size_t)i + c] = (float)((double)pos4[4 * (size_t)i + c] + u4[4 * (size_t)i + c] * dt);
+        pos4[4 * (size_t)i + 3] = 0.0f;
+    }
+}
+
+/* GPU_kernel.cu:3237-3254 compute_posa_kernel: the centre is 0.5 (b + c)
+ * with the sum taken in float. */
+void orc_segment_centres(const float *posc4, const float *posb4, float *posa4, uint32_t n)
+{
+    uint32_t i;
+    for (i = 0; i < n; i++) {
+        int c;
+        for (c = 0; c < 3; c++) {
+            float sum = posb4[4 * (size_t)i + c] + posc4[4 * (size_t)i + c];
+            posa4[4 * (size_t)i + c] = (float)(0.5 * (double)sum);
+        }
+    }
+}
+
+/* GPU_kernel.cu:3143-3180 compute_vort_kernel: dw = kappa (b - a) - w with
+ * b - a taken in float. */
+void orc_vortices_from_segments(const float *posa4, const float *posb4, const double *wx, const double *wy,
+                                const double *wz, double *dwx, double *dwy, double *dwz, const double *kappa,
+                                uint32_t n)
+{
+    uint32_t i;
+    for (i = 0; i < n; i++) {
+        double sx = (double)(posb4[4 * (size_t)i] - posa4[4 * (size_t)i]);
+        double sy = (double)(posb4[4 * (size_t)i + 1] - posa4[4 * (size_t)i + 1]);
+        double sz = (double)(posb4[4 * (size_t)i + 2] - posa4[4 * (size_t)i + 2]);
+        dwx[i] = kappa[i] * sx - wx[i];
+        dwy[i] = kappa[i] * sy - wy[i];
+        dwz[i] = kappa[i] * sz - wz[i];
+    }
+}
+
+/* GPU_kernel.cu:3327-3438 particle_gaussian_kernel / ParticleGaussianSmooth:
+ * o[t] = sum_p e^{-d^2 / (2 nu^2)} i[p] / sum_p e^{-d^2 / (2 nu^2)} over the
+ * sorted particles p of the hash cells within +-2 of t's cell (total weight
+ * below 1e-12 divides by 1e24).  pos4 in caller order, pos4_sorted and i_*
+ * in hash order; d is taken between origin-shifted float coordinates. */
+void orc_gaussian_smooth(const uint32_t *start, const uint32_t *end, const float *pos4,
+                         const float *pos4_sorted, double nu, double *ox, double *oy, double *oz,
+                         const double *ix, const double *iy, const double *iz, double h, int g, uint32_t n,
+                         const double origin[3])
+{
+    const double invh = 1.0 / h;
+    int64_t t;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (t = 0; t < (int64_t)n; t++) {
+        float ax = (float)((double)pos4[4 * t] - origin[0]);
+        float ay = (float)((double)pos4[4 * t + 1] - origin[1]);
+        float az = (float)((double)pos4[4 * t + 2] - origin[2]);
+        int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+        double sx = 0, sy = 0, sz = 0, tw = 0;
+        int kk, jj, ii;
+        for (kk = k - 2; kk <= k + 2; kk++)
+            for (jj = j - 2; jj <= j + 2; jj++)
+                for (ii = i - 2; ii <= i + 2; ii++) {
+                    uint32_t cell, p;
+                    if (!(kk >= 0 && kk < g && jj >= 0 && jj < g && ii >= 0 && ii < g)) continue;
+                    cell = (uint32_t)(((int64_t)kk * g + jj) * g + ii);
+                    for (p = start[cell]; p < end[cell]; p++) {
+                        float bx, by, bz;
+                        double dx, dy, dz, d2, wgt;
+                        if (p >= n) continue;
+                        bx = (float)((double)pos4_sorted[4 * (size_t)p] - origin[0]);
+                        by = (float)((double)pos4_sorted[4 * (size_t)p + 1] - origin[1]);
+                        bz = (float)((double)pos4_sorted[4 * (size_t)p + 2] - origin[2]);
+                        dx = (double)(float)(bx - ax);
+                        dy = (double)(float)(by - ay);
+                        dz = (double)(float)(bz - az);
+                        d2 = dx * dx + dy * dy + dz * dz;
+                        wgt = exp(-d2 / (2 * nu * nu));
+                        tw += wgt;
+                        sx += wgt * ix[p];
+                        sy += wgt * iy[p];
+                        sz += wgt * iz[p];
+                    }
+                }
+        if (tw < 1e-12) tw = 1e+24;
+        ox[t] = sx / tw;
+        oy[t] = sy / tw;
+        oz[t] = sz / tw;
+    }
+}

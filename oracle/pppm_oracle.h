@@ -110,6 +110,19 @@ void orc_direct_sum(uint32_t n, const float *vort_pos4, const double *wx,
                     double *ux, double *uy, double *uz);
 void orc_reference_domain(const float *pos4, uint32_t n, uint32_t g,
                           double origin_out[3], double *L_out);
+/* ---- time step of the vortex-segment method -------------------------- */
+void orc_segments_from_vortices(const float *cent4, float *enda4, float *endb4, const double *wx,
+                                const double *wy, const double *wz, double *kappa, double h, uint32_t n);
+void orc_move_points(float *pos4, double dt, const double *u4, uint32_t n);
+void orc_segment_centres(const float *posc4, const float *posb4, float *posa4, uint32_t n);
+void orc_vortices_from_segments(const float *posa4, const float *posb4, const double *wx, const double *wy,
+                                const double *wz, double *dwx, double *dwy, double *dwz, const double *kappa,
+                                uint32_t n);
+void orc_gaussian_smooth(const uint32_t *start, const uint32_t *end, const float *pos4,
+                         const float *pos4_sorted, double nu, double *ox, double *oy, double *oz,
+                         const double *ix, const double *iy, const double *iz, double h, int g, uint32_t n,
+                         const double origin[3]);
+
 int  orc_num_threads(void);
 
 #ifdef __cplusplus

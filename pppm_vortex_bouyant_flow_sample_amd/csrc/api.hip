@@ -195,6 +195,9 @@ struct pppm_ctx {
     // tracer batches: FP64 velocities of one batch, [3][capT]
     double* utr = nullptr;
     uint32_t capT = 0;
+    // values being smoothed, FP64 in hash order
+    double4* sval = nullptr;
+    uint32_t capS = 0;
     uint32_t *cstart = nullptr, *cend = nullptr, *coff = nullptr, *scan_scr = nullptr;
     uint32_t* d_strays = nullptr;
     uint32_t* h_strays = nullptr;
@@ -304,6 +307,7 @@ struct pppm_ctx {
         dev_free(somg);
         dev_free(m2p);
         dev_free(utr);
+        dev_free(sval);
         dev_free(cstart);
         dev_free(cend);
         dev_free(coff);
@@ -814,6 +818,26 @@ extern "C" pppm_status pppm_evaluate_tracers(pppm_ctx* c, uint32_t m, float* pos
         PPPM_LAUNCH_CHECK();
     }
     return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_smooth_vorticity(pppm_ctx* c, const double* vx, const double* vy, const double* vz,
+                                             uint32_t v_stride, double* ox, double* oy, double* oz,
+                                             uint32_t o_stride, double nu) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (!c->have_vort) FAIL(PPPM_ERR_STATE, "set_vortices first");
+    if (v_stride == 0 || o_stride == 0) FAIL(PPPM_ERR_INVALID, "stride");
+    if (!(nu > 0)) FAIL(PPPM_ERR_INVALID, "nu must be > 0");
+    const uint32_t n = c->n;
+    if (n == 0) return PPPM_OK;
+    if (!vx || !vy || !vz || !ox || !oy || !oz) FAIL(PPPM_ERR_INVALID, "null array");
+    if (n > c->capS || !c->sval) {
+        dev_free(c->sval);
+        c->capS = 0;
+        TRY(dev_alloc(&c->sval, n));
+        c->capS = n;
+    }
+    return launch_gaussian_smooth(c->spos, c->coff, c->vs.vs, n, c->pg, nu, vx, vy, vz, v_stride, ox, oy, oz,
+                                  o_stride, c->sval, c->st);
 }
 
 extern "C" pppm_status pppm_solve_self(pppm_ctx* c, uint32_t n, const float* pos4, const void* wx, const void* wy,

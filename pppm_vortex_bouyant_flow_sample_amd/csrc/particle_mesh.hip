@@ -352,3 +352,41 @@ extern "C" void BiotSavartInterpolateFarField(float4* pos, double* far_u, double
                        p_v, p_w, 1.0 / cell_h, (int)dimx, (int)dimy, (int)dimz, num_particles, origin.x, origin.y,
                        origin.z);
 }
+
+// ---- time step (VortexSimulationSystemImp.cpp:90-125) ----------------------
+extern "C" void ComputePosb(float4* posa, float4* posc, float4* posb, double* vortx, double* vorty, double* vortz,
+                            double* k, double h, uint32_t num_particles) {
+    (void)pppm_segments_from_vortices((const float*)posa, (float*)posc, (float*)posb, vortx, vorty, vortz, k, h,
+                                      num_particles, nullptr);
+}
+
+extern "C" void move_particle(float4* pos, double dt, double4* U, uint32_t num_particles) {
+    (void)pppm_move_points((float*)pos, (const double*)U, dt, num_particles, nullptr);
+}
+
+extern "C" void ComputePosa(float4* posc, float4* posb, float4* posa, uint32_t num_particles) {
+    (void)pppm_segment_centres((const float*)posc, (const float*)posb, (float*)posa, num_particles, nullptr);
+}
+
+extern "C" void ComputeVortex(float4* posa, float4* posb, double* vortx, double* vorty, double* vortz,
+                              double* dvortx, double* dvorty, double* dvortz, double* k, uint32_t num_particles) {
+    (void)pppm_vortices_from_segments((const float*)posa, (const float*)posb, vortx, vorty, vortz, dvortx, dvorty,
+                                      dvortz, k, num_particles, nullptr);
+}
+
+extern "C" void VectorAdd(double* A, double* B, double* C, double a, double b, uint32_t num) {
+    (void)pppm_vector_add(A, B, C, a, b, num, nullptr);
+}
+
+extern "C" void AddURotation(double4* U, double* u, double* v, double* w, uint32_t num) {
+    (void)pppm_add_velocity((double*)U, u, v, w, num, nullptr);
+}
+
+extern "C" void ParticleGaussianSmooth(uint32_t* start, uint32_t* end, float4* pos, float4* pos_sort, double nu,
+                                       double dt, double* o_vortx, double* o_vorty, double* o_vortz,
+                                       double* i_vortx, double* i_vorty, double* i_vortz, double h, uint3 gdim,
+                                       uint32_t N, double4 origin) {
+    (void)dt;
+    launch_gaussian_smooth_tables(start, end, pos, pos_sort, nu, o_vortx, o_vorty, o_vortz, i_vortx, i_vorty,
+                                  i_vortz, h, gdim, N, origin, 0);
+}

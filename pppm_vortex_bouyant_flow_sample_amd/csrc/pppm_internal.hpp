@@ -108,4 +108,17 @@ pppm_status launch_octant_order(const uint32_t* ks, const float4* pos, uint32_t 
 pppm_status launch_m2p_near(const NearArgs& a, hipStream_t st);
 pppm_status launch_count_pairs(const NearArgs& a, unsigned long long* d_total, hipStream_t st);
 
+// ---- time step (timestep.hip) ----------------------------------------------
+// Gaussian smoothing of caller-order FP64 values over the sorted vortices
+// (spos, coff, idx of the context's hash); scratch holds n double4.
+pppm_status launch_gaussian_smooth(const float4* spos, const uint32_t* coff, const uint32_t* idx, uint32_t n,
+                                   const PathGeom& pg, double nu, const double* vx, const double* vy,
+                                   const double* vz, uint32_t v_stride, double* ox, double* oy, double* oz,
+                                   uint32_t o_stride, double4* scratch, hipStream_t st);
+// The same over a caller's start/end tables and sorted arrays.
+void launch_gaussian_smooth_tables(const uint32_t* start, const uint32_t* end, const float4* pos,
+                                   const float4* pos_sort, double nu, double* ox, double* oy, double* oz,
+                                   const double* ix, const double* iy, const double* iz, double h, uint3 hd,
+                                   uint32_t n, double4 origin, hipStream_t st);
+
 }  // namespace pppm

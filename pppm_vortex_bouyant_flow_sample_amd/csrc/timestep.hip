@@ -1,0 +1,325 @@
+// Time step of the vortex-segment method (VortexSimulationSystemImp.cpp:90-125
+// solveTimeStep): each vortex becomes a short segment, both ends move with
+// the flow, and the segment turns back into a vortex whose stretching is
+// smoothed over its neighbours.  Kernels of GPU_kernel.cu:3118-3254 (segment
+// ends, moves, centres, vorticity change), 3327-3438 (Gaussian smoothing),
+// 3442-3460 (VectorAdd) and 3589-3606 (AddURotation), with the reference's
+// arithmetic: FP64 except where it rounds through float (positions, the
+// float differences of positions).  FP contraction is off in this file so
+// every product and sum rounds as the reference writes it.
+#include "pppm_internal.hpp"
+
+#pragma clang fp contract(off)
+
+#define FAIL(code, msg)                                  \
+    do {                                                 \
+        pppm_set_last_error((msg), __FILE__, __LINE__);  \
+        return (code);                                   \
+    } while (0)
+
+namespace pppm {
+
+// compute_posb_kernel (GPU_kernel.cu:3182-3227): half = w/2 when |w| < h
+// (kappa = 1), else w h / (2 |w|) (kappa = |w| / h); endb = c + half,
+// enda = c - half.  The ends' w fields are left as they were.
+__global__ void __launch_bounds__(256) k_segments_from_vortices(const float4* __restrict__ cent,
+                                                                float* __restrict__ enda, float* __restrict__ endb,
+                                                                const double* __restrict__ wx,
+                                                                const double* __restrict__ wy,
+                                                                const double* __restrict__ wz,
+                                                                double* __restrict__ kappa, double h, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s[3] = {wx[i], wy[i], wz[i]};
+    const double l = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const bool shortseg = l < h;
+    kappa[i] = shortseg ? 1.0 : l / h;
+    const float4 c = cent[i];
+    const double a[3] = {(double)c.x, (double)c.y, (double)c.z};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double half = shortseg ? 0.5 * s[q] : 0.5 * s[q] / l * h;
+        endb[4 * (size_t)i + q] = (float)(a[q] + half);
+        enda[4 * (size_t)i + q] = (float)(a[q] - half);
+    }
+}
+
+// move_particle_kernel (GPU_kernel.cu:3118-3133): x += u dt in FP64, stored
+// as float with w = 0.
+__device__ inline float4 moved(float4 p, double4 u, double dt) {
+    return make_float4((float)((double)p.x + u.x * dt), (float)((double)p.y + u.y * dt),
+                       (float)((double)p.z + u.z * dt), 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_move_points(float4* __restrict__ pos, const double4* __restrict__ u,
+                                                     double dt, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos[i] = moved(pos[i], u[i], dt);
+}
+
+// compute_posa_kernel (GPU_kernel.cu:3237-3247): centre = 0.5 (b + c), the
+// sum in float.
+__device__ inline float midpoint(float b, float c) { return (float)(0.5 * (double)(b + c)); }
+
+__global__ void __launch_bounds__(256) k_segment_centres(const float4* __restrict__ posc,
+                                                         const float4* __restrict__ posb, float* __restrict__ posa,
+                                                         uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 b = posb[i], c = posc[i];
+    posa[4 * (size_t)i] = midpoint(b.x, c.x);
+    posa[4 * (size_t)i + 1] = midpoint(b.y, c.y);
+    posa[4 * (size_t)i + 2] = midpoint(b.z, c.z);
+}
+
+// applyFlow (VortexSimulationSystemImp.cpp:277-300) without a boundary: both
+// ends moved (two move_particle launches) and the centre re-formed
+// (ComputePosa) in one pass, with the same per-element arithmetic.
+__global__ void __launch_bounds__(256) k_apply_flow(float4* __restrict__ enda, float4* __restrict__ endb,
+                                                    float* __restrict__ cent, const double4* __restrict__ ua,
+                                                    const double4* __restrict__ ub, double dt, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = moved(enda[i], ua[i], dt), b = moved(endb[i], ub[i], dt);
+    enda[i] = a;
+    endb[i] = b;
+    cent[4 * (size_t)i] = midpoint(b.x, a.x);
+    cent[4 * (size_t)i + 1] = midpoint(b.y, a.y);
+    cent[4 * (size_t)i + 2] = midpoint(b.z, a.z);
+}
+
+// compute_vort_kernel (GPU_kernel.cu:3143-3173): dw = kappa (b - a) - w, the
+// difference of positions in float.
+__global__ void __launch_bounds__(256) k_vortices_from_segments(
+    const float4* __restrict__ posa, const float4* __restrict__ posb, const double* __restrict__ wx,
+    const double* __restrict__ wy, const double* __restrict__ wz, double* __restrict__ dwx, double* __restrict__ dwy,
+    double* __restrict__ dwz, const double* __restrict__ kappa, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = posa[i], b = posb[i];
+    const double k = kappa[i];
+    dwx[i] = k * (double)(b.x - a.x) - wx[i];
+    dwy[i] = k * (double)(b.y - a.y) - wy[i];
+    dwz[i] = k * (double)(b.z - a.z) - wz[i];
+}
+
+// vectorAdd_kernel (GPU_kernel.cu:3442-3450): C = a A + b B.
+__global__ void __launch_bounds__(256) k_axpby(const double* A, const double* B, double* C, double a, double b,
+                                               uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) C[i] = a * A[i] + b * B[i];
+}
+
+// add_u_rotation_kernel (GPU_kernel.cu:3589-3599): U.xyz += (u, v, w).
+__global__ void __launch_bounds__(256) k_add_velocity(double4* __restrict__ U, const double* __restrict__ u,
+                                                      const double* __restrict__ v, const double* __restrict__ w,
+                                                      uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 x = U[i];
+    x.x += u[i];
+    x.y += v[i];
+    x.z += w[i];
+    U[i] = x;
+}
+
+// ---- Gaussian smoothing over the context's hash ---------------------------
+// The values to smooth, gathered into hash order as FP64 (x, y, z, 0).
+__global__ void __launch_bounds__(256) k_gather_values(const uint32_t* __restrict__ idx,
+                                                       const double* __restrict__ vx, const double* __restrict__ vy,
+                                                       const double* __restrict__ vz, uint32_t stride, uint32_t n,
+                                                       double4* __restrict__ sv) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t j = (size_t)idx[t] * stride;
+    sv[t] = make_double4(vx[j], vy[j], vz[j], 0.0);
+}
+
+// particle_gaussian_kernel (GPU_kernel.cu:3327-3411), one thread per particle
+// in hash order: the weighted mean of the sorted values over the hash cells
+// within +-2 of the particle's cell, weight e^{-d^2 / (2 nu^2)} with d
+// between origin-shifted float coordinates, written to the caller's index.
+// The five cells of one (y, z) row are contiguous in hash order, so each row
+// is one range of the cell offsets; rows run in the reference's order (z, y,
+// then x), so within a cell only the order of its particles differs.
+__global__ void __launch_bounds__(256) k_gaussian_smooth(const float4* __restrict__ spos,
+                                                         const double4* __restrict__ sv,
+                                                         const uint32_t* __restrict__ coff,
+                                                         const uint32_t* __restrict__ idx, uint32_t n, uint32_t g,
+                                                         double invh, double den, double* __restrict__ ox,
+                                                         double* __restrict__ oy, double* __restrict__ oz,
+                                                         uint32_t o_stride) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 a = spos[t];
+    const int gi = (int)g;
+    // the particle's cell, clamped to [-3, g + 2] (a cell farther out has no
+    // in-grid cell within +-2, as a cell at -3 or g + 2 has none)
+    auto cell = [&](float q) { return (int)fmin(fmax(floor((double)q * invh), -3.0), (double)gi + 2.0); };
+    const int i = cell(a.x), j = cell(a.y), k = cell(a.z);
+    const int i0 = max(i - 2, 0), i1 = min(i + 2, gi - 1);
+    double sx = 0, sy = 0, sz = 0, tw = 0;
+    if (i0 <= i1)
+        for (int kk = max(k - 2, 0); kk <= min(k + 2, gi - 1); kk++)
+            for (int jj = max(j - 2, 0); jj <= min(j + 2, gi - 1); jj++) {
+                const size_t row = ((size_t)kk * g + (size_t)jj) * g;
+                const uint32_t p1 = coff[row + (size_t)i1 + 1];
+                for (uint32_t p = coff[row + (size_t)i0]; p < p1; p++) {
+                    const float4 b = spos[p];
+                    const double dx = (double)(b.x - a.x), dy = (double)(b.y - a.y), dz = (double)(b.z - a.z);
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double w = exp(-d2 / den);
+                    const double4 v = sv[p];
+                    tw += w;
+                    sx += w * v.x;
+                    sy += w * v.y;
+                    sz += w * v.z;
+                }
+            }
+    if (tw < 1e-12) tw = 1e+24;
+    const size_t o = (size_t)idx[t] * o_stride;
+    ox[o] = sx / tw;
+    oy[o] = sy / tw;
+    oz[o] = sz / tw;
+}
+
+pppm_status launch_gaussian_smooth(const float4* spos, const uint32_t* coff, const uint32_t* idx, uint32_t n,
+                                   const PathGeom& pg, double nu, const double* vx, const double* vy,
+                                   const double* vz, uint32_t v_stride, double* ox, double* oy, double* oz,
+                                   uint32_t o_stride, double4* scratch, hipStream_t st) {
+    if (n == 0) return PPPM_OK;
+    hipLaunchKernelGGL(k_gather_values, dim3(div_up(n, 256)), dim3(256), 0, st, idx, vx, vy, vz, v_stride, n,
+                       scratch);
+    PPPM_LAUNCH_CHECK();
+    const double den = 2 * nu * nu;
+    hipLaunchKernelGGL(k_gaussian_smooth, dim3(div_up(n, 256)), dim3(256), 0, st, spos, scratch, coff, idx, n,
+                       pg.hash.g, pg.invh, den, ox, oy, oz, o_stride);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+// particle_gaussian_kernel over a caller's start/end tables and sorted
+// arrays, as ParticleGaussianSmooth (GPU_kernel.cu:3414-3438) takes them.
+__global__ void __launch_bounds__(256) k_gaussian_smooth_tables(
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const float4* __restrict__ pos,
+    const float4* __restrict__ pos_sort, double den, double* __restrict__ ox, double* __restrict__ oy,
+    double* __restrict__ oz, const double* __restrict__ ix, const double* __restrict__ iy,
+    const double* __restrict__ iz, double invh, uint3 hd, uint32_t n, double orx, double ory, double orz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 A = pos[t];
+    const float ax = (float)((double)A.x - orx), ay = (float)((double)A.y - ory), az = (float)((double)A.z - orz);
+    auto cell = [&](float q, uint32_t d) {
+        return (int)fmin(fmax(floor((double)q * invh), -3.0), (double)d + 2.0);
+    };
+    const int i = cell(ax, hd.x), j = cell(ay, hd.y), k = cell(az, hd.z);
+    double sx = 0, sy = 0, sz = 0, tw = 0;
+    for (int kk = k - 2; kk <= k + 2; kk++)
+        for (int jj = j - 2; jj <= j + 2; jj++)
+            for (int ii = i - 2; ii <= i + 2; ii++) {
+                if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x) continue;
+                const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                for (uint32_t p = start[q]; p < end[q]; p++) {
+                    if (p >= n) continue;
+                    const float4 B = pos_sort[p];
+                    const float bx = (float)((double)B.x - orx), by = (float)((double)B.y - ory),
+                                bz = (float)((double)B.z - orz);
+                    const double dx = (double)(bx - ax), dy = (double)(by - ay), dz = (double)(bz - az);
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const double w = exp(-d2 / den);
+                    tw += w;
+                    sx += w * ix[p];
+                    sy += w * iy[p];
+                    sz += w * iz[p];
+                }
+            }
+    if (tw < 1e-12) tw = 1e+24;
+    ox[t] = sx / tw;
+    oy[t] = sy / tw;
+    oz[t] = sz / tw;
+}
+
+void launch_gaussian_smooth_tables(const uint32_t* start, const uint32_t* end, const float4* pos,
+                                   const float4* pos_sort, double nu, double* ox, double* oy, double* oz,
+                                   const double* ix, const double* iy, const double* iz, double h, uint3 hd,
+                                   uint32_t n, double4 origin, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gaussian_smooth_tables, dim3(div_up(n, 256)), dim3(256), 0, st, start, end, pos, pos_sort,
+                       2 * nu * nu, ox, oy, oz, ix, iy, iz, 1.0 / h, hd, n, origin.x, origin.y, origin.z);
+}
+
+}  // namespace pppm
+
+using namespace pppm;
+
+extern "C" pppm_status pppm_segments_from_vortices(const float* cent4, float* enda4, float* endb4, const double* wx,
+                                                   const double* wy, const double* wz, double* kappa, double h,
+                                                   uint32_t n, void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!cent4 || !enda4 || !endb4 || !wx || !wy || !wz || !kappa) FAIL(PPPM_ERR_INVALID, "null array");
+    if (!(h > 0)) FAIL(PPPM_ERR_INVALID, "h must be > 0");
+    hipLaunchKernelGGL(k_segments_from_vortices, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream,
+                       (const float4*)cent4, enda4, endb4, wx, wy, wz, kappa, h, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_move_points(float* pos4, const double* u4, double dt, uint32_t n, void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!pos4 || !u4) FAIL(PPPM_ERR_INVALID, "null array");
+    hipLaunchKernelGGL(k_move_points, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream, (float4*)pos4,
+                       (const double4*)u4, dt, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_segment_centres(const float* enda4, const float* endb4, float* cent4, uint32_t n,
+                                            void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!enda4 || !endb4 || !cent4) FAIL(PPPM_ERR_INVALID, "null array");
+    hipLaunchKernelGGL(k_segment_centres, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream,
+                       (const float4*)enda4, (const float4*)endb4, cent4, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_apply_flow(float* enda4, float* endb4, float* cent4, const double* ua4,
+                                       const double* ub4, double dt, uint32_t n, void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!enda4 || !endb4 || !cent4 || !ua4 || !ub4) FAIL(PPPM_ERR_INVALID, "null array");
+    hipLaunchKernelGGL(k_apply_flow, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream, (float4*)enda4,
+                       (float4*)endb4, cent4, (const double4*)ua4, (const double4*)ub4, dt, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_vortices_from_segments(const float* enda4, const float* endb4, const double* wx,
+                                                   const double* wy, const double* wz, double* dwx, double* dwy,
+                                                   double* dwz, const double* kappa, uint32_t n, void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!enda4 || !endb4 || !wx || !wy || !wz || !dwx || !dwy || !dwz || !kappa) FAIL(PPPM_ERR_INVALID, "null array");
+    hipLaunchKernelGGL(k_vortices_from_segments, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream,
+                       (const float4*)enda4, (const float4*)endb4, wx, wy, wz, dwx, dwy, dwz, kappa, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_vector_add(const double* A, const double* B, double* C, double a, double b, uint32_t n,
+                                       void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!A || !B || !C) FAIL(PPPM_ERR_INVALID, "null array");
+    hipLaunchKernelGGL(k_axpby, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream, A, B, C, a, b, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_add_velocity(double* U4, const double* u, const double* v, const double* w, uint32_t n,
+                                         void* hip_stream) {
+    if (n == 0) return PPPM_OK;
+    if (!U4 || !u || !v || !w) FAIL(PPPM_ERR_INVALID, "null array");
+    hipLaunchKernelGGL(k_add_velocity, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)hip_stream, (double4*)U4, u,
+                       v, w, n);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
+}

@@ -146,6 +146,51 @@ def write_frame(directory, frame, pos_ptr, n, w_ptr=None, stream=None):
                                  vp(stream)), "write_frame")
 
 
+def _vp(p):
+    return p if isinstance(p, C.c_void_p) or p is None else C.c_void_p(p)
+
+
+# ---- time step of the vortex-segment method (device pointers) ----------------
+def segments_from_vortices(cent, enda, endb, w3, kappa, h, n, stream=None):
+    """pppm_segments_from_vortices (ComputePosb)."""
+    check(lib().pppm_segments_from_vortices(_vp(cent), _vp(enda), _vp(endb), *[_vp(q) for q in w3], _vp(kappa),
+                                            C.c_double(h), C.c_uint32(n), _vp(stream)), "segments_from_vortices")
+
+
+def move_points(pos, u4, dt, n, stream=None):
+    """pppm_move_points (move_particle)."""
+    check(lib().pppm_move_points(_vp(pos), _vp(u4), C.c_double(dt), C.c_uint32(n), _vp(stream)), "move_points")
+
+
+def segment_centres(enda, endb, cent, n, stream=None):
+    """pppm_segment_centres (ComputePosa)."""
+    check(lib().pppm_segment_centres(_vp(enda), _vp(endb), _vp(cent), C.c_uint32(n), _vp(stream)),
+          "segment_centres")
+
+
+def apply_flow(enda, endb, cent, ua4, ub4, dt, n, stream=None):
+    """pppm_apply_flow: both ends moved and the centre re-formed."""
+    check(lib().pppm_apply_flow(_vp(enda), _vp(endb), _vp(cent), _vp(ua4), _vp(ub4), C.c_double(dt), C.c_uint32(n),
+                                _vp(stream)), "apply_flow")
+
+
+def vortices_from_segments(enda, endb, w3, dw3, kappa, n, stream=None):
+    """pppm_vortices_from_segments (ComputeVortex)."""
+    check(lib().pppm_vortices_from_segments(_vp(enda), _vp(endb), *[_vp(q) for q in w3], *[_vp(q) for q in dw3],
+                                            _vp(kappa), C.c_uint32(n), _vp(stream)), "vortices_from_segments")
+
+
+def vector_add(A, B, Cp, a, b, n, stream=None):
+    """pppm_vector_add: C = a A + b B."""
+    check(lib().pppm_vector_add(_vp(A), _vp(B), _vp(Cp), C.c_double(a), C.c_double(b), C.c_uint32(n), _vp(stream)),
+          "vector_add")
+
+
+def add_velocity(U4, u3, n, stream=None):
+    """pppm_add_velocity: U.xyz += u3."""
+    check(lib().pppm_add_velocity(_vp(U4), *[_vp(q) for q in u3], C.c_uint32(n), _vp(stream)), "add_velocity")
+
+
 def read_frame(path):
     """A frame file as an (n, 4) float32 array (x, y, z, w)."""
     return np.fromfile(path, dtype=np.float32).reshape(-1, 4)
@@ -218,6 +263,13 @@ class Context:
     def evaluate_tracers(self, m, pos_buf, vel_buf, batch, dt=0.0, move=False):
         check(lib().pppm_evaluate_tracers(self.h, C.c_uint32(m), pos_buf, vel_buf, C.c_uint32(batch),
                                           C.c_double(dt), C.c_int32(1 if move else 0)), "evaluate_tracers")
+
+    def smooth_vorticity(self, v3, o3, nu=0.1, v_stride=1, o_stride=1):
+        """pppm_smooth_vorticity: o = Gaussian mean of v over the hashed
+        vortices' +-2-cell neighbourhoods (v3, o3: three device pointers)."""
+        check(lib().pppm_smooth_vorticity(self.h, *[_vp(q) for q in v3], C.c_uint32(v_stride),
+                                          *[_vp(q) for q in o3], C.c_uint32(o_stride), C.c_double(nu)),
+              "smooth_vorticity")
 
     def stats(self):
         s = Stats()
