@@ -353,6 +353,25 @@ pppm_status pppm_smooth_vorticity(pppm_ctx *ctx, const double *vx, const double 
                                   uint32_t v_stride, double *ox, double *oy, double *oz, uint32_t o_stride,
                                   double nu);
 
+/* setDomain (BiotSavartSolverImp.cpp:378-409, run by sortVort and
+ * computeFarFieldBuffer, 731-770) over device positions: the bounding box
+ * by a device reduction (min and max are exact), then origin = box centre -
+ * L/2, h = L / g with L = 3 * the largest extent -- the same numbers as
+ * pppm_reference_domain on a host copy.  Synchronises the context stream. */
+pppm_status pppm_domain_device(pppm_ctx *ctx, const float *pos4, uint32_t n, double origin_out[3], double *h_out);
+/* One step of solveTimeStep (VortexSimulationSystemImp.cpp:90-125) for a
+ * flow without boundary, heat or density: convertVortToSeg (segment ends
+ * about the centres, length min(|w|, seg_h)); computeURotation (the centres
+ * hashed on their own domain, the far field, the velocity at each end with
+ * the own vortex excluded, in the context's precision); applyFlow (forward
+ * Euler of both ends, centres re-formed); convertSegToVort; computeStretching
+ * (w + dw smoothed with pppm_smooth_vorticity's Gaussian over the new
+ * centres, nu as given; the reference uses seg_h = 0.1, nu = 0.1).  cent4
+ * and w (FP64, n each) are updated in place; the context moves its domain
+ * and keeps 128 bytes per vortex of step state. */
+pppm_status pppm_time_step(pppm_ctx *ctx, uint32_t n, float *cent4, double *wx, double *wy, double *wz, double dt,
+                           double seg_h, double nu);
+
 /* ---- timing on the library's streams --------------------------------------- */
 typedef struct pppm_timer pppm_timer;
 pppm_status pppm_timer_create(pppm_timer **out);

@@ -249,6 +249,8 @@ def direct_sum(vort_pos4, omega, eval_pos4=None, kernel=KERNEL_EXPCUBIC, core_a=
 
 
 def reference_domain(pos4, g):
+    """setDomain (BiotSavartSolverImp.cpp:378-409): (origin, L), the domain
+    length L = 3 * the largest extent; the cell size is L / g."""
     pos4 = _f32(pos4)
     o = (C.c_double * 3)()
     L = C.c_double(0)
@@ -323,3 +325,27 @@ def gaussian_smooth(params, pos4, values, nu=0.1):
                               _p(out[2]), _p(iv[0]), _p(iv[1]), _p(iv[2]), C.c_double(float(np.float32(params.hf))),
                               C.c_int(g), C.c_uint32(n), o)
     return np.stack(out)
+
+
+def time_step(cent4, omega, dt, seg_h, nu, g, K=2, kernel=KERNEL_EXPCUBIC, core_a=None, tol=1e-12,
+              max_cycles=80):
+    """solveTimeStep without boundary, heat or density
+    (VortexSimulationSystemImp.cpp:90-125): segment ends, the end velocities
+    by the PPPM path on the centres' domain (own vortex excluded), forward
+    Euler of the ends, new centres, vorticity change, Gaussian smoothing on the
+    new centres' domain.  Returns (centres (N, 4), omega (3, N), stages dict)."""
+    cent4 = _f32(cent4)
+    n = cent4.shape[0]
+    enda, endb, kappa = segments_from_vortices(cent4, omega, seg_h)
+    o, L = reference_domain(cent4, g)
+    p = make_params(g, L / g, o, K=K, kernel=kernel, core_a=core_a, tol=tol, max_cycles=max_cycles)
+    ub, _ = pppm_eval(p, cent4, omega, eval_pos4=endb, other_end4=cent4)
+    ua, _ = pppm_eval(p, cent4, omega, eval_pos4=enda, other_end4=cent4)
+    u4a, u4b = np.zeros((n, 4)), np.zeros((n, 4))
+    u4a[:, :3], u4b[:, :3] = ua.T, ub.T
+    a1, b1 = move_points(enda, dt, u4a), move_points(endb, dt, u4b)
+    c1 = segment_centres(a1, b1, cent4)
+    dw = vortices_from_segments(a1, b1, omega, kappa)
+    o2, L2 = reference_domain(c1, g)
+    w1 = gaussian_smooth(make_params(g, L2 / g, o2), c1, np.asarray(omega) + dw, nu)
+    return c1, w1, {"ua": ua, "ub": ub, "domain": (o, L / g), "domain_after": (o2, L2 / g)}

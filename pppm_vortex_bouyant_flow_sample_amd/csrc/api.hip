@@ -198,6 +198,12 @@ struct pppm_ctx {
     // values being smoothed, FP64 in hash order
     double4* sval = nullptr;
     uint32_t capS = 0;
+    // time step: segment ends, kappa, vorticity change [3][n], end velocities
+    float4 *seg_a = nullptr, *seg_b = nullptr;
+    double *kappa = nullptr, *dvort = nullptr;
+    double4 *vel_a = nullptr, *vel_b = nullptr;
+    uint32_t capStep = 0;
+    float* bbox = nullptr;
     uint32_t *cstart = nullptr, *cend = nullptr, *coff = nullptr, *scan_scr = nullptr;
     uint32_t* d_strays = nullptr;
     uint32_t* h_strays = nullptr;
@@ -308,6 +314,13 @@ struct pppm_ctx {
         dev_free(m2p);
         dev_free(utr);
         dev_free(sval);
+        dev_free(seg_a);
+        dev_free(seg_b);
+        dev_free(kappa);
+        dev_free(dvort);
+        dev_free(vel_a);
+        dev_free(vel_b);
+        dev_free(bbox);
         dev_free(cstart);
         dev_free(cend);
         dev_free(coff);
@@ -401,16 +414,10 @@ extern "C" void pppm_params_reference_preset(pppm_params* p, uint32_t g, int32_t
     p->mg_max_cycles = 1;
 }
 
-extern "C" pppm_status pppm_reference_domain(const float* pos, uint32_t n, uint32_t g, double origin_out[3],
-                                             double* h_out) {
-    if (!pos || n == 0 || g == 0 || !origin_out || !h_out) FAIL(PPPM_ERR_INVALID, "reference_domain args");
-    double mx = pos[0], Mx = pos[0], my = pos[1], My = pos[1], mz = pos[2], Mz = pos[2];
-    for (uint32_t i = 1; i < n; i++) {
-        double x = pos[4 * (size_t)i], y = pos[4 * (size_t)i + 1], z = pos[4 * (size_t)i + 2];
-        if (x < mx) mx = x; else if (x > Mx) Mx = x;
-        if (y < my) my = y; else if (y > My) My = y;
-        if (z < mz) mz = z; else if (z > Mz) Mz = z;
-    }
+// setDomain's formula (BiotSavartSolverImp.cpp:378-409): L = 3 * the largest
+// extent, origin = bbox centre - L/2, h = L / g.
+static pppm_status domain_from_extent(double mx, double Mx, double my, double My, double mz, double Mz, uint32_t g,
+                                      double origin_out[3], double* h_out) {
     double L = Mx - mx;
     if (My - my > L) L = My - my;
     if (Mz - mz > L) L = Mz - mz;
@@ -421,6 +428,19 @@ extern "C" pppm_status pppm_reference_domain(const float* pos, uint32_t n, uint3
     *h_out = L / (double)g;
     if (!(L > 0)) FAIL(PPPM_ERR_INVALID, "degenerate particle extent");
     return PPPM_OK;
+}
+
+extern "C" pppm_status pppm_reference_domain(const float* pos, uint32_t n, uint32_t g, double origin_out[3],
+                                             double* h_out) {
+    if (!pos || n == 0 || g == 0 || !origin_out || !h_out) FAIL(PPPM_ERR_INVALID, "reference_domain args");
+    double mx = pos[0], Mx = pos[0], my = pos[1], My = pos[1], mz = pos[2], Mz = pos[2];
+    for (uint32_t i = 1; i < n; i++) {
+        double x = pos[4 * (size_t)i], y = pos[4 * (size_t)i + 1], z = pos[4 * (size_t)i + 2];
+        if (x < mx) mx = x; else if (x > Mx) Mx = x;
+        if (y < my) my = y; else if (y > My) My = y;
+        if (z < mz) mz = z; else if (z > Mz) Mz = z;
+    }
+    return domain_from_extent(mx, Mx, my, My, mz, Mz, g, origin_out, h_out);
 }
 
 extern "C" pppm_status pppm_create(const pppm_params* p, pppm_ctx** out) {
@@ -838,6 +858,67 @@ extern "C" pppm_status pppm_smooth_vorticity(pppm_ctx* c, const double* vx, cons
     }
     return launch_gaussian_smooth(c->spos, c->coff, c->vs.vs, n, c->pg, nu, vx, vy, vz, v_stride, ox, oy, oz,
                                   o_stride, c->sval, c->st);
+}
+
+extern "C" pppm_status pppm_domain_device(pppm_ctx* c, const float* pos4, uint32_t n, double origin_out[3],
+                                          double* h_out) {
+    if (!c || !pos4 || n == 0 || !origin_out || !h_out) FAIL(PPPM_ERR_INVALID, "domain_device args");
+    if (!c->bbox) TRY(dev_alloc(&c->bbox, bbox_scratch_floats()));
+    TRY(launch_bbox((const float4*)pos4, n, c->bbox, c->st));
+    float b[6];
+    PPPM_HIP_TRY(hipMemcpyAsync(b, c->bbox + bbox_scratch_floats() - 6, sizeof b, hipMemcpyDeviceToHost, c->st));
+    PPPM_HIP_TRY(hipStreamSynchronize(c->st));
+    return domain_from_extent(b[0], b[3], b[1], b[4], b[2], b[5], c->g, origin_out, h_out);
+}
+
+extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, double* wx, double* wy, double* wz,
+                                      double dt, double seg_h, double nu) {
+    if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
+    if (n == 0) return PPPM_OK;
+    if (!cent4 || !wx || !wy || !wz) FAIL(PPPM_ERR_INVALID, "null array");
+    if (!(seg_h > 0) || !(nu > 0)) FAIL(PPPM_ERR_INVALID, "seg_h and nu must be > 0");
+    if (n > c->capStep) {
+        for (float4** q : {&c->seg_a, &c->seg_b}) dev_free(*q);
+        dev_free(c->kappa);
+        dev_free(c->dvort);
+        dev_free(c->vel_a);
+        dev_free(c->vel_b);
+        c->capStep = 0;
+        TRY(dev_alloc(&c->seg_a, n));
+        TRY(dev_alloc(&c->seg_b, n));
+        TRY(dev_alloc(&c->kappa, n));
+        TRY(dev_alloc(&c->dvort, 3 * (size_t)n));
+        TRY(dev_alloc(&c->vel_a, n));
+        TRY(dev_alloc(&c->vel_b, n));
+        c->capStep = n;
+    }
+    hipStream_t st = c->st;
+    float* a = (float*)c->seg_a;
+    float* b = (float*)c->seg_b;
+    double* ua = (double*)c->vel_a;
+    double* ub = (double*)c->vel_b;
+    double* dw[3] = {c->dvort, c->dvort + n, c->dvort + 2 * (size_t)n};
+    double* w[3] = {wx, wy, wz};
+    double o[3], h;
+    // convertVortToSeg
+    TRY(pppm_segments_from_vortices(cent4, a, b, wx, wy, wz, c->kappa, seg_h, n, st));
+    // computeURotation: the centres hashed on their own domain, the far
+    // field, then both ends in segment mode (the own vortex excluded)
+    TRY(pppm_domain_device(c, cent4, n, o, &h));
+    TRY(pppm_set_domain(c, o, h));
+    TRY(pppm_set_vortices(c, n, cent4, wx, wy, wz, 1, PPPM_F64));
+    TRY(pppm_compute_far_field(c));
+    TRY(pppm_evaluate(c, n, b, cent4, ub, ub + 1, ub + 2, 4, PPPM_F64));
+    TRY(pppm_evaluate(c, n, a, cent4, ua, ua + 1, ua + 2, 4, PPPM_F64));
+    // applyFlow, convertSegToVort
+    TRY(pppm_apply_flow(a, b, cent4, ua, ub, dt, n, st));
+    TRY(pppm_vortices_from_segments(a, b, wx, wy, wz, dw[0], dw[1], dw[2], c->kappa, n, st));
+    // computeStretching: w + dw smoothed over the new centres' neighbourhoods
+    for (int q = 0; q < 3; q++) TRY(pppm_vector_add(w[q], dw[q], dw[q], 1.0, 1.0, n, st));
+    TRY(pppm_domain_device(c, cent4, n, o, &h));
+    TRY(pppm_set_domain(c, o, h));
+    TRY(pppm_set_vortices(c, n, cent4, dw[0], dw[1], dw[2], 1, PPPM_F64));
+    return pppm_smooth_vorticity(c, dw[0], dw[1], dw[2], 1, wx, wy, wz, 1, nu);
 }
 
 extern "C" pppm_status pppm_solve_self(pppm_ctx* c, uint32_t n, const float* pos4, const void* wx, const void* wy,

@@ -121,4 +121,9 @@ void launch_gaussian_smooth_tables(const uint32_t* start, const uint32_t* end, c
                                    const double* ix, const double* iy, const double* iz, double h, uint3 hd,
                                    uint32_t n, double4 origin, hipStream_t st);
 
+// Bounding box of device positions into scratch (bbox_scratch_floats()
+// floats): the six results (min xyz, max xyz) at scratch + 6 * 1024.
+size_t bbox_scratch_floats();
+pppm_status launch_bbox(const float4* pos, uint32_t n, float* scratch, hipStream_t st);
+
 }  // namespace pppm

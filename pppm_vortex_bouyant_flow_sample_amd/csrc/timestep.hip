@@ -7,6 +7,8 @@
 // arithmetic: FP64 except where it rounds through float (positions, the
 // float differences of positions).  FP contraction is off in this file so
 // every product and sum rounds as the reference writes it.
+#include <algorithm>
+
 #include "pppm_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -247,6 +249,62 @@ void launch_gaussian_smooth_tables(const uint32_t* start, const uint32_t* end, c
     if (n == 0) return;
     hipLaunchKernelGGL(k_gaussian_smooth_tables, dim3(div_up(n, 256)), dim3(256), 0, st, start, end, pos, pos_sort,
                        2 * nu * nu, ox, oy, oz, ix, iy, iz, 1.0 / h, hd, n, origin.x, origin.y, origin.z);
+}
+
+// ---- bounding box of device positions --------------------------------------
+// Min and max of x, y, z (exact in any order): per-block partials, then one
+// block over the partials; out = (min x, min y, min z, max x, max y, max z).
+constexpr int kBoxThreads = 256, kBoxMaxBlocks = 1024;
+
+__device__ inline void box_block_reduce(float v[6], float* out) {
+    __shared__ float red[6][kBoxThreads];
+    for (int q = 0; q < 6; q++) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int s = kBoxThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int q = 0; q < 6; q++) {
+                const float o = red[q][threadIdx.x + s];
+                red[q][threadIdx.x] = q < 3 ? fminf(red[q][threadIdx.x], o) : fmaxf(red[q][threadIdx.x], o);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kBoxThreads) k_bbox(const float4* __restrict__ pos, uint32_t n,
+                                                      float* __restrict__ part) {
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * kBoxThreads + threadIdx.x; i < n; i += gridDim.x * kBoxThreads) {
+        const float4 p = pos[i];
+        v[0] = fminf(v[0], p.x);
+        v[1] = fminf(v[1], p.y);
+        v[2] = fminf(v[2], p.z);
+        v[3] = fmaxf(v[3], p.x);
+        v[4] = fmaxf(v[4], p.y);
+        v[5] = fmaxf(v[5], p.z);
+    }
+    box_block_reduce(v, part + 6 * (size_t)blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBoxThreads) k_bbox_final(const float* __restrict__ part, uint32_t nb,
+                                                            float* __restrict__ out) {
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t b = threadIdx.x; b < nb; b += kBoxThreads)
+        for (int q = 0; q < 6; q++) v[q] = q < 3 ? fminf(v[q], part[6 * b + q]) : fmaxf(v[q], part[6 * b + q]);
+    box_block_reduce(v, out);
+}
+
+size_t bbox_scratch_floats() { return 6 * (size_t)kBoxMaxBlocks + 6; }
+
+pppm_status launch_bbox(const float4* pos, uint32_t n, float* scratch, hipStream_t st) {
+    if (n == 0) return PPPM_ERR_INVALID;
+    const uint32_t nb = std::min<uint32_t>(div_up(n, kBoxThreads * 8), kBoxMaxBlocks);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBoxThreads), 0, st, pos, n, scratch);
+    PPPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBoxThreads), 0, st, scratch, nb,
+                       scratch + 6 * (size_t)kBoxMaxBlocks);
+    PPPM_LAUNCH_CHECK();
+    return PPPM_OK;
 }
 
 }  // namespace pppm

@@ -271,6 +271,18 @@ class Context:
                                           *[_vp(q) for q in o3], C.c_uint32(o_stride), C.c_double(nu)),
               "smooth_vorticity")
 
+    def domain_device(self, pos_ptr, n):
+        """pppm_domain_device: (origin [3], h) from device positions."""
+        o = (C.c_double * 3)()
+        h = C.c_double(0)
+        check(lib().pppm_domain_device(self.h, _vp(pos_ptr), C.c_uint32(n), o, C.byref(h)), "domain_device")
+        return [o[0], o[1], o[2]], h.value
+
+    def time_step(self, n, cent_ptr, w3, dt, seg_h=0.1, nu=0.1):
+        """pppm_time_step: one solveTimeStep on device state (in place)."""
+        check(lib().pppm_time_step(self.h, C.c_uint32(n), _vp(cent_ptr), *[_vp(q) for q in w3], C.c_double(dt),
+                                   C.c_double(seg_h), C.c_double(nu)), "time_step")
+
     def stats(self):
         s = Stats()
         check(lib().pppm_get_stats(self.h, C.byref(s)), "get_stats")

@@ -45,7 +45,8 @@ def test_particle_mesh_functions(native, oracle, tmp_path):
     assert os.path.exists(EXE), "build() first"
     N, g, K = 30000, 32, 1
     pos, w = scenes.ring(N, seed=31)
-    o, h = oracle.reference_domain(pos, g)
+    o, L = oracle.reference_domain(pos, g)
+    h = L / g
     with open(tmp_path / "in.bin", "wb") as f:
         f.write(np.array([N, g], np.uint32).tobytes())
         f.write(np.array([K], np.int32).tobytes())

@@ -112,7 +112,8 @@ def test_smooth_vorticity_context(native, oracle, g):
     DB = native.DeviceBuffer
     n = 30_000
     pos, v = _smooth_scene(n, 21)
-    o, h = oracle.reference_domain(pos[: n - 8], g)
+    o, L = oracle.reference_domain(pos[: n - 8], g)
+    h = L / g
     # a few points outside the grid take the edge and outside paths
     pos[n - 8:, :3] = np.array([[o[0] - 0.01, o[1], o[2]], [o[0] + 3, o[1], o[2]]] * 4, np.float32)
     print(f"[timestep] smooth_vorticity n={n} g={g}", flush=True)
@@ -165,7 +166,8 @@ def test_reference_named_time_step(native, oracle, tmp_path):
     u4a[:, :3], u4b[:, :3] = ua.T, ub.T
     ma, mb = oracle.move_points(ea, dt, u4a), oracle.move_points(eb, dt, u4b)
     mc = oracle.segment_centres(ma, mb, cent)
-    o, h = oracle.reference_domain(mc, g)
+    o, L = oracle.reference_domain(mc, g)
+    h = L / g
     with open(tmp_path / "in.bin", "wb") as f:
         f.write(np.array([n, g], np.uint32).tobytes())
         f.write(np.array(list(o) + [h, seg_h, dt], np.float64).tobytes())
@@ -198,7 +200,59 @@ def test_reference_named_time_step(native, oracle, tmp_path):
     _within_ulp(b1, oracle.move_points(b0, dt, u4b))
     np.testing.assert_array_equal(c1, oracle.segment_centres(a1, b1, cent))
     np.testing.assert_allclose(dw, oracle.vortices_from_segments(a1, b1, w, k0), rtol=1e-15, atol=1e-18)
-    o2, h2 = oracle.reference_domain(c1, g)
+    o2, L = oracle.reference_domain(c1, g)
+    h2 = L / g
     ref = oracle.gaussian_smooth(oracle.make_params(g, h, o), c1, w + dw, 0.1)
     np.testing.assert_allclose(w1, ref, rtol=1e-12, atol=1e-14)
     print(f"  domain of the GPU centres vs the oracle's: {np.max(np.abs(np.subtract(o2, o))):.3g}", flush=True)
+
+
+def test_domain_device_matches_host(native):
+    pos, _ = scenes.ring(50_001, seed=5)
+    pos = np.array(pos, np.float32)
+    ctx = native.Context(native.config_params(32))
+    d = native.DeviceBuffer.from_array(pos)
+    print("[timestep] domain_device n=50001", flush=True)
+    o, h = ctx.domain_device(d.ptr, pos.shape[0])
+    o2, h2 = native.reference_domain(pos, 32)
+    assert o == list(o2) and h == h2
+
+
+@pytest.mark.parametrize("scale,steps", [(1e3, 2), (1e4, 1)], ids=["kappa1", "kappa5"])
+def test_time_step_against_oracle(native, oracle, scale, steps):
+    """pppm_time_step (mixed precision) against the oracle's FP64 step, with
+    the reference's seg_h = nu = 0.1.  Strengths 0.05 (kappa = 1, segments
+    0.05 long) and 0.5 (kappa = 5, segments 0.1 long): segments far longer
+    than a float ulp of the positions, so the step is well conditioned (a
+    velocity error of 1e-5 moves the result by < 1e-5; with strengths of
+    5e-5 the 5e-5-long segments would be float rounding noise).  dt moves the
+    fastest end about one cell."""
+    DB = native.DeviceBuffer
+    n, g = 20_000, 32
+    pos, w = scenes.ring(n, seed=33)
+    cent = np.array(pos, np.float32)
+    w = np.array(w, np.float64) * scale
+    o, L = oracle.reference_domain(cent, g)
+    dt = 0.5 * (L / g) / (scale * 1.0)
+    p = native.config_params(g, precision=1, mg_tol=1e-9)
+    ctx = native.Context(p)
+    d_c = DB.from_array(cent)
+    d_w = [DB.from_array(np.ascontiguousarray(w[c])) for c in range(3)]
+    c_ref, w_ref = cent, w
+    for s in range(steps):
+        print(f"[timestep] time_step scale={scale} step {s}", flush=True)
+        ctx.time_step(n, d_c.ptr, [q.ptr for q in d_w], dt, 0.1, 0.1)
+        native.synchronize()
+        c_prev, w_prev = c_ref, w_ref
+        c_ref, w_ref, _ = oracle.time_step(c_ref, w_ref, dt, 0.1, 0.1, g, K=p.K, kernel=p.kernel,
+                                           core_a=p.core_a, tol=1e-12)
+        c_gpu = d_c.to_array(np.float32).reshape(n, 4)
+        w_gpu = np.stack([q.to_array(np.float64) for q in d_w])
+        disp_ref = (c_ref[:, :3] - c_prev[:, :3]).astype(np.float64)
+        disp = (c_gpu[:, :3] - c_prev[:, :3]).astype(np.float64)
+        e_disp = np.linalg.norm(disp - disp_ref) / np.linalg.norm(disp_ref)
+        e_w = np.linalg.norm(w_gpu - w_ref) / np.linalg.norm(w_ref - w_prev)
+        print(f"  displacement rel {e_disp:.3g}  vorticity rel to its change {e_w:.3g}", flush=True)
+        assert e_disp < 2e-5 and e_w < 2e-5
+        # continue both from the GPU's state so the comparison stays one step deep
+        c_ref, w_ref = c_gpu, w_gpu

@@ -106,7 +106,8 @@ def test_gaussian_smooth_brute_force(oracle, nu):
     n = 1500
     pos = np.zeros((n, 4), np.float32)
     pos[:, :3] = (0.5 + 0.12 * rng.standard_normal((n, 3))).astype(np.float32)
-    o, h = oracle.reference_domain(pos, g)
+    o, L = oracle.reference_domain(pos, g)
+    h = L / g
     params = oracle.make_params(g, h, o)
     v = rng.standard_normal((3, n))
     got = oracle.gaussian_smooth(params, pos, v, nu)
@@ -120,7 +121,8 @@ def test_gaussian_smooth_constant_and_linear(oracle):
     n = 3000
     pos = np.zeros((n, 4), np.float32)
     pos[:, :3] = rng.random((n, 3)).astype(np.float32)
-    o, h = oracle.reference_domain(pos, g)
+    o, L = oracle.reference_domain(pos, g)
+    h = L / g
     params = oracle.make_params(g, h, o)
     c = np.ones((3, n)) * np.array([[1.5], [-2.0], [0.25]])
     np.testing.assert_allclose(oracle.gaussian_smooth(params, pos, c), c, rtol=1e-14)
