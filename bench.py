@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paths", default="served,mixed,fp32")
     ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--time-steps", type=int, default=3,
+                    help="simulation time steps (pppm_time_step) timed after the paths; 0 skips")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run an identity gather of 16M float4 rows (4 B index + 16 B read, 16 B written "
                          "per row) three times, the known-size kernel that HBM counters are scaled by")
@@ -216,6 +218,32 @@ def cpu_baseline():
     return out
 
 
+def run_time_step(N, pos, w, g, u_ref, steps):
+    """solveTimeStep on device state (pppm_time_step, mixed precision, the
+    reference's seg_h = nu = 0.1) on the workload's fixed grid ([0,1)^3,
+    h = 1/g), dt = 0.5 h / max|u| from a velocity evaluation of the same
+    state.  One untimed step first (allocation)."""
+    n = pos.shape[0]
+    p = N.config_params(g, precision=N.PREC_MIXED, mg_tol=MIXED_TOL, mg_max_cycles=40)
+    ctx = N.Context(p)
+    dt = 0.5 * (1.0 / g) / float(np.abs(u_ref).max())
+    cent = N.DeviceBuffer.from_array(np.ascontiguousarray(pos, np.float32))
+    wd = [N.DeviceBuffer.from_array(np.ascontiguousarray(w[c], np.float64)) for c in range(3)]
+    ctx.time_step(n, cent.ptr, [b.ptr for b in wd], dt, 0.1, 0.1, keep_domain=True)
+    N.synchronize()
+    lat = []
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        ctx.time_step(n, cent.ptr, [b.ptr for b in wd], dt, 0.1, 0.1, keep_domain=True)
+        N.synchronize()
+        lat.append(time.perf_counter() - s0)
+    ctx.close()
+    ms = 1e3 * float(np.mean(lat))
+    return {"ms_per_step": ms, "vortex_steps_per_s": n / (ms * 1e-3), "steps": steps, "dt": dt, "seg_h": 0.1,
+            "nu": 0.1, "precision": "mixed", "domain": "fixed [0,1)^3, h = 1/g",
+            "integrator": "forward Euler of segment ends (solveTimeStep)"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -294,6 +322,11 @@ def main():
         }
     if "fp32" in res:
         out["vs_plain_fp32"] = value / out["fp32_path"]["value"] if head != "fp32" else 1.0
+    if a.time_steps > 0:
+        u_ref = (res.get("mixed") or res.get("fp32") or res.get("served"))["u_last"]
+        log(f"[bench] time step: {a.time_steps} steps of pppm_time_step")
+        out["time_step"] = run_time_step(N, pos, w, g, u_ref, a.time_steps)
+        log(f"[bench]   {out['time_step']['ms_per_step']:.1f} ms/step")
     if a.copy_calibration:
         rows = 16 << 20
         idx = N.DeviceBuffer.from_array(np.arange(rows, dtype=np.uint32))

@@ -367,10 +367,15 @@ pppm_status pppm_domain_device(pppm_ctx *ctx, const float *pos4, uint32_t n, dou
  * Euler of both ends, centres re-formed); convertSegToVort; computeStretching
  * (w + dw smoothed with pppm_smooth_vorticity's Gaussian over the new
  * centres, nu as given; the reference uses seg_h = 0.1, nu = 0.1).  cent4
- * and w (FP64, n each) are updated in place; the context moves its domain
- * and keeps 128 bytes per vortex of step state. */
+ * and w (FP64, n each) are updated in place; the context keeps 128 bytes per
+ * vortex of step state.  The domain follows the reference: before each
+ * hashing (velocities, smoothing) it is set from the centres by
+ * pppm_domain_device.  flags PPPM_STEP_KEEP_DOMAIN keeps the context's
+ * domain instead (a fixed grid, as a long run on a given grid uses; vortices
+ * outside it are handled as by pppm_set_vortices). */
+enum { PPPM_STEP_KEEP_DOMAIN = 1 };
 pppm_status pppm_time_step(pppm_ctx *ctx, uint32_t n, float *cent4, double *wx, double *wy, double *wz, double dt,
-                           double seg_h, double nu);
+                           double seg_h, double nu, int32_t flags);
 
 /* ---- timing on the library's streams --------------------------------------- */
 typedef struct pppm_timer pppm_timer;

@@ -328,16 +328,20 @@ def gaussian_smooth(params, pos4, values, nu=0.1):
 
 
 def time_step(cent4, omega, dt, seg_h, nu, g, K=2, kernel=KERNEL_EXPCUBIC, core_a=None, tol=1e-12,
-              max_cycles=80):
+              max_cycles=80, domain=None):
     """solveTimeStep without boundary, heat or density
     (VortexSimulationSystemImp.cpp:90-125): segment ends, the end velocities
     by the PPPM path on the centres' domain (own vortex excluded), forward
     Euler of the ends, new centres, vorticity change, Gaussian smoothing on the
-    new centres' domain.  Returns (centres (N, 4), omega (3, N), stages dict)."""
+    new centres' domain.  domain=(origin, h) keeps that fixed domain instead.
+    Returns (centres (N, 4), omega (3, N), stages dict)."""
     cent4 = _f32(cent4)
     n = cent4.shape[0]
     enda, endb, kappa = segments_from_vortices(cent4, omega, seg_h)
-    o, L = reference_domain(cent4, g)
+    if domain is None:
+        o, L = reference_domain(cent4, g)
+    else:
+        o, L = domain[0], domain[1] * g
     p = make_params(g, L / g, o, K=K, kernel=kernel, core_a=core_a, tol=tol, max_cycles=max_cycles)
     ub, _ = pppm_eval(p, cent4, omega, eval_pos4=endb, other_end4=cent4)
     ua, _ = pppm_eval(p, cent4, omega, eval_pos4=enda, other_end4=cent4)
@@ -346,6 +350,6 @@ def time_step(cent4, omega, dt, seg_h, nu, g, K=2, kernel=KERNEL_EXPCUBIC, core_
     a1, b1 = move_points(enda, dt, u4a), move_points(endb, dt, u4b)
     c1 = segment_centres(a1, b1, cent4)
     dw = vortices_from_segments(a1, b1, omega, kappa)
-    o2, L2 = reference_domain(c1, g)
+    o2, L2 = reference_domain(c1, g) if domain is None else (o, L)
     w1 = gaussian_smooth(make_params(g, L2 / g, o2), c1, np.asarray(omega) + dw, nu)
     return c1, w1, {"ua": ua, "ub": ub, "domain": (o, L / g), "domain_after": (o2, L2 / g)}

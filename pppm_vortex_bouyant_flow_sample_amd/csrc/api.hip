@@ -872,11 +872,13 @@ extern "C" pppm_status pppm_domain_device(pppm_ctx* c, const float* pos4, uint32
 }
 
 extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, double* wx, double* wy, double* wz,
-                                      double dt, double seg_h, double nu) {
+                                      double dt, double seg_h, double nu, int32_t flags) {
     if (!c) FAIL(PPPM_ERR_INVALID, "null ctx");
     if (n == 0) return PPPM_OK;
     if (!cent4 || !wx || !wy || !wz) FAIL(PPPM_ERR_INVALID, "null array");
     if (!(seg_h > 0) || !(nu > 0)) FAIL(PPPM_ERR_INVALID, "seg_h and nu must be > 0");
+    if (flags & ~PPPM_STEP_KEEP_DOMAIN) FAIL(PPPM_ERR_INVALID, "unknown step flags");
+    const bool move_domain = !(flags & PPPM_STEP_KEEP_DOMAIN);
     if (n > c->capStep) {
         for (float4** q : {&c->seg_a, &c->seg_b}) dev_free(*q);
         dev_free(c->kappa);
@@ -904,8 +906,10 @@ extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, dou
     TRY(pppm_segments_from_vortices(cent4, a, b, wx, wy, wz, c->kappa, seg_h, n, st));
     // computeURotation: the centres hashed on their own domain, the far
     // field, then both ends in segment mode (the own vortex excluded)
-    TRY(pppm_domain_device(c, cent4, n, o, &h));
-    TRY(pppm_set_domain(c, o, h));
+    if (move_domain) {
+        TRY(pppm_domain_device(c, cent4, n, o, &h));
+        TRY(pppm_set_domain(c, o, h));
+    }
     TRY(pppm_set_vortices(c, n, cent4, wx, wy, wz, 1, PPPM_F64));
     TRY(pppm_compute_far_field(c));
     TRY(pppm_evaluate(c, n, b, cent4, ub, ub + 1, ub + 2, 4, PPPM_F64));
@@ -915,8 +919,10 @@ extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, dou
     TRY(pppm_vortices_from_segments(a, b, wx, wy, wz, dw[0], dw[1], dw[2], c->kappa, n, st));
     // computeStretching: w + dw smoothed over the new centres' neighbourhoods
     for (int q = 0; q < 3; q++) TRY(pppm_vector_add(w[q], dw[q], dw[q], 1.0, 1.0, n, st));
-    TRY(pppm_domain_device(c, cent4, n, o, &h));
-    TRY(pppm_set_domain(c, o, h));
+    if (move_domain) {
+        TRY(pppm_domain_device(c, cent4, n, o, &h));
+        TRY(pppm_set_domain(c, o, h));
+    }
     TRY(pppm_set_vortices(c, n, cent4, dw[0], dw[1], dw[2], 1, PPPM_F64));
     return pppm_smooth_vorticity(c, dw[0], dw[1], dw[2], 1, wx, wy, wz, 1, nu);
 }

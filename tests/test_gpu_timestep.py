@@ -256,3 +256,32 @@ def test_time_step_against_oracle(native, oracle, scale, steps):
         assert e_disp < 2e-5 and e_w < 2e-5
         # continue both from the GPU's state so the comparison stays one step deep
         c_ref, w_ref = c_gpu, w_gpu
+
+
+def test_time_step_keep_domain(native, oracle):
+    """PPPM_STEP_KEEP_DOMAIN: the context's own fixed grid for both hashings."""
+    DB = native.DeviceBuffer
+    n, g = 20_000, 32
+    pos, w = scenes.ring(n, seed=35)
+    cent = np.array(pos, np.float32)
+    w = np.array(w, np.float64) * 1e3
+    o, L = oracle.reference_domain(cent, g)
+    h = L / g
+    dt = 0.5 * h / 1e3
+    p = native.config_params(g, h=h, origin=o, precision=1, mg_tol=1e-9)
+    ctx = native.Context(p)
+    d_c = DB.from_array(cent)
+    d_w = [DB.from_array(np.ascontiguousarray(w[c])) for c in range(3)]
+    print("[timestep] time_step keep_domain", flush=True)
+    ctx.time_step(n, d_c.ptr, [q.ptr for q in d_w], dt, 0.1, 0.1, keep_domain=True)
+    native.synchronize()
+    c_ref, w_ref, _ = oracle.time_step(cent, w, dt, 0.1, 0.1, g, K=p.K, kernel=p.kernel, core_a=p.core_a,
+                                       tol=1e-12, domain=(o, h))
+    c_gpu = d_c.to_array(np.float32).reshape(n, 4)
+    w_gpu = np.stack([q.to_array(np.float64) for q in d_w])
+    disp_ref = (c_ref[:, :3] - cent[:, :3]).astype(np.float64)
+    disp = (c_gpu[:, :3] - cent[:, :3]).astype(np.float64)
+    e_disp = np.linalg.norm(disp - disp_ref) / np.linalg.norm(disp_ref)
+    e_w = np.linalg.norm(w_gpu - w_ref) / np.linalg.norm(w_ref - w)
+    print(f"  displacement rel {e_disp:.3g}  vorticity rel to its change {e_w:.3g}", flush=True)
+    assert e_disp < 2e-5 and e_w < 2e-5
