@@ -780,7 +780,7 @@ extern "C" pppm_status pppm_evaluate(pppm_ctx* c, uint32_t m, const float* eval_
     hipStream_t st = c->st;
     TRY(c->ensure_eval(m, other_end4 != nullptr));
     const bool ok8 = c->octant_keys();
-    TRY(launch_calc_keys((const float4*)eval_pos4, m, c->pg.hash, c->es.k0, nullptr, st, ok8));
+    TRY(launch_calc_keys((const float4*)eval_pos4, m, c->pg.hash, c->es.k0, nullptr, st, ok8, true));
     TRY(radix_sort_pairs(c->es.k0, c->es.v0, c->es.k1, c->es.v1, m, key_bits_for(c->G) + (ok8 ? 3u : 0u), true,
                          c->es.scratch, &c->es.ks, &c->es.vs, st));
     TRY(launch_reorder_eval(c->es.vs, (const float4*)eval_pos4, (const float4*)other_end4, m, c->pg, c->epos,

@@ -16,21 +16,22 @@ constexpr int kRadix = 1 << kRadixBits;
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_calc_keys(const float4* __restrict__ pos, uint32_t n, uint32_t g, float hf,
                                                    float ox, float oy, float oz, uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ idx, bool octant) {
+                                                   uint32_t* __restrict__ idx, bool octant, bool clamp) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 p = pos[i];
-    keys[i] = octant ? hash_cell_octant_key(p.x, p.y, p.z, g, hf, ox, oy, oz)
-                     : hash_cell_key(p.x, p.y, p.z, g, hf, ox, oy, oz);
+    keys[i] = clamp    ? eval_cell_key(p.x, p.y, p.z, g, hf, ox, oy, oz, octant)
+              : octant ? hash_cell_octant_key(p.x, p.y, p.z, g, hf, ox, oy, oz)
+                       : hash_cell_key(p.x, p.y, p.z, g, hf, ox, oy, oz);
     if (idx) idx[i] = i;
 }
 
 pppm_status launch_calc_keys(const float4* pos, uint32_t n, const HashGeom& hg, uint32_t* keys, uint32_t* idx,
-                             hipStream_t st, bool octant) {
+                             hipStream_t st, bool octant, bool clamp) {
     if (n == 0) return PPPM_OK;
     if (octant && (uint64_t)hg.g * hg.g * hg.g * 8 > ((uint64_t)1 << 32)) return PPPM_ERR_INVALID;
     hipLaunchKernelGGL(k_calc_keys, dim3(div_up(n, 256)), dim3(256), 0, st, pos, n, hg.g, hg.hf, hg.of[0], hg.of[1],
-                       hg.of[2], keys, idx, octant);
+                       hg.of[2], keys, idx, octant, clamp);
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }

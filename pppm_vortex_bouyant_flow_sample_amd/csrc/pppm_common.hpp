@@ -81,6 +81,28 @@ __device__ inline uint32_t hash_cell_octant_key(float px, float py, float pz, ui
     return ((uz * g + uy) * g + ux) * 8u + o;
 }
 
+// Sort key of an evaluation point: hash_cell_octant_key (or the plain cell
+// key times 8 when !octant is wanted by the caller) with each cell coordinate
+// clamped to [0, g) instead of wrapped, so a point outside the grid sorts
+// beside the edge cells nearest to it.  Evaluation order is internal: results
+// go back to the caller's order, and the vortex hash keeps the reference's
+// wrapped keys.
+__device__ inline uint32_t eval_cell_key(float px, float py, float pz, uint32_t g, float hf, float ox, float oy,
+                                         float oz, bool octant) {
+    float qx = __fdiv_rn(__fsub_rn(px, ox), hf);
+    float qy = __fdiv_rn(__fsub_rn(py, oy), hf);
+    float qz = __fdiv_rn(__fsub_rn(pz, oz), hf);
+    float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+    const int gm = (int)g - 1;
+    uint32_t ux = (uint32_t)min(max(f2i_sat(fx), 0), gm);
+    uint32_t uy = (uint32_t)min(max(f2i_sat(fy), 0), gm);
+    uint32_t uz = (uint32_t)min(max(f2i_sat(fz), 0), gm);
+    uint32_t key = (uz * g + uy) * g + ux;
+    if (!octant) return key;
+    uint32_t o = (uint32_t)(qx - fx >= 0.5f) | ((uint32_t)(qy - fy >= 0.5f) << 1) | ((uint32_t)(qz - fz >= 0.5f) << 2);
+    return key * 8u + o;
+}
+
 // Position relative to the double origin, rounded to float
 // (GPU_kernel.cu:252-255: float4 ppos; ppos.x -= origin.x).
 __device__ inline float shift_coord(float p, double o) { return __double2float_rn((double)p - o); }

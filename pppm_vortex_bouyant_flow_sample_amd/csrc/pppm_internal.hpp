@@ -30,8 +30,9 @@ uint32_t key_bits_for(uint64_t ncells);
 // octant: key = 8 * cell + octant of the point within its cell (the
 // halves in x, y and z of the hash quotient; needs 8 * cells <= 2^32), so a
 // sort orders each cell's points by octant.
+// clamp: evaluation-point keys (eval_cell_key), cells clamped into the grid.
 pppm_status launch_calc_keys(const float4* pos, uint32_t n, const HashGeom& hg, uint32_t* keys, uint32_t* idx,
-                             hipStream_t st, bool octant = false);
+                             hipStream_t st, bool octant = false, bool clamp = false);
 // Sorts (keys, vals) by key, stable; on return *keys_out/*vals_out point at
 // whichever buffer pair holds the result.  vals_identity: the input values
 // are 0..n-1 and vals is not read.
