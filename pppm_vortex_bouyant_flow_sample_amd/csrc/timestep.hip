@@ -8,6 +8,7 @@
 // float differences of positions).  FP contraction is off in this file so
 // every product and sum rounds as the reference writes it.
 #include <algorithm>
+#include <cmath>
 
 #include "pppm_internal.hpp"
 
@@ -144,41 +145,112 @@ __global__ void __launch_bounds__(256) k_gather_values(const uint32_t* __restric
 // between origin-shifted float coordinates, written to the caller's index.
 // The five cells of one (y, z) row are contiguous in hash order, so each row
 // is one range of the cell offsets; rows run in the reference's order (z, y,
-// then x), so within a cell only the order of its particles differs.
+// then x), so within a cell only the order of its particles differs.  The
+// weight is e^{d^2 * nid} with nid = -1 / (2 nu^2) (a product in place of
+// the reference's quotient: the exponent differs by at most one rounding),
+// and the squared distance and the sums use fused multiply-adds.
+// DEG > 0: every exponent lies in [-xm, 0] with xm = 28 h^2 |nid| (two
+// particles within +-2 cells are less than 3 h apart per axis), and e^x is
+// its Taylor series to x^DEG, DEG the least of 5, 7, 9, 11 whose remainder
+// xm^(DEG+1) / (DEG+1)! is below 1e-17 (chosen on the host), in place of
+// the library exp (DEG = 0).
+template <int DEG>
+__device__ inline double gauss_weight(double x) {
+    if (DEG == 0) return exp(x);
+    // 1/k! for k = 0..11
+    constexpr double c[12] = {1.0,         1.0,          0.5,           1.0 / 6.0,      1.0 / 24.0,
+                              1.0 / 120.0, 1.0 / 720.0,  1.0 / 5040.0,  1.0 / 40320.0,  1.0 / 362880.0,
+                              1.0 / 3628800.0, 1.0 / 39916800.0};
+    double p = c[DEG];
+#pragma unroll
+    for (int q = DEG - 1; q >= 0; q--) p = fma(p, x, c[q]);
+    return p;
+}
+
+__device__ inline int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Cells in a wave's union box up to which the wave walks the union together.
+constexpr int64_t kSmoothUnion = 250;
+
+// A wave's lanes are consecutive particles in hash order, mostly of one or
+// two cells.  When the union of their +-2-cell boxes holds at most
+// kSmoothUnion cells, the wave walks that union cell by cell with uniform
+// (broadcast) loads, and each lane adds only the cells of its own box: the
+// same cells in the same order as its own walk, so the same sums.  Otherwise
+// each lane walks its own box.
+template <int DEG>
 __global__ void __launch_bounds__(256) k_gaussian_smooth(const float4* __restrict__ spos,
                                                          const double4* __restrict__ sv,
                                                          const uint32_t* __restrict__ coff,
                                                          const uint32_t* __restrict__ idx, uint32_t n, uint32_t g,
-                                                         double invh, double den, double* __restrict__ ox,
+                                                         double invh, double nid, double* __restrict__ ox,
                                                          double* __restrict__ oy, double* __restrict__ oz,
                                                          uint32_t o_stride) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const float4 a = spos[t];
+    const bool live = t < n;
+    // lanes past the end take the last particle (n > 0) and write nothing
+    const float4 a = spos[live ? t : n - 1];
     const int gi = (int)g;
     // the particle's cell, clamped to [-3, g + 2] (a cell farther out has no
     // in-grid cell within +-2, as a cell at -3 or g + 2 has none)
     auto cell = [&](float q) { return (int)fmin(fmax(floor((double)q * invh), -3.0), (double)gi + 2.0); };
     const int i = cell(a.x), j = cell(a.y), k = cell(a.z);
-    const int i0 = max(i - 2, 0), i1 = min(i + 2, gi - 1);
     double sx = 0, sy = 0, sz = 0, tw = 0;
-    if (i0 <= i1)
-        for (int kk = max(k - 2, 0); kk <= min(k + 2, gi - 1); kk++)
-            for (int jj = max(j - 2, 0); jj <= min(j + 2, gi - 1); jj++) {
+    auto pair = [&](uint32_t p, bool use) {
+        const float4 b = spos[p];
+        const double dx = (double)(b.x - a.x), dy = (double)(b.y - a.y), dz = (double)(b.z - a.z);
+        const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double w = gauss_weight<DEG>(d2 * nid);
+        const double4 v = sv[p];
+        if (use) {
+            tw += w;
+            sx = fma(w, v.x, sx);
+            sy = fma(w, v.y, sy);
+            sz = fma(w, v.z, sz);
+        }
+    };
+    const int X0 = __builtin_amdgcn_readfirstlane(max(wave_min(i) - 2, 0));
+    const int X1 = __builtin_amdgcn_readfirstlane(min(wave_max(i) + 2, gi - 1));
+    const int Y0 = __builtin_amdgcn_readfirstlane(max(wave_min(j) - 2, 0));
+    const int Y1 = __builtin_amdgcn_readfirstlane(min(wave_max(j) + 2, gi - 1));
+    const int Z0 = __builtin_amdgcn_readfirstlane(max(wave_min(k) - 2, 0));
+    const int Z1 = __builtin_amdgcn_readfirstlane(min(wave_max(k) + 2, gi - 1));
+    const int64_t vol = (X1 >= X0 && Y1 >= Y0 && Z1 >= Z0)
+                            ? (int64_t)(X1 - X0 + 1) * (Y1 - Y0 + 1) * (Z1 - Z0 + 1) : 0;
+    if (vol <= kSmoothUnion) {
+        for (int kk = Z0; kk <= Z1; kk++) {
+            const bool kin = abs(kk - k) <= 2;
+            for (int jj = Y0; jj <= Y1; jj++) {
+                const bool rin = kin && abs(jj - j) <= 2;
                 const size_t row = ((size_t)kk * g + (size_t)jj) * g;
-                const uint32_t p1 = coff[row + (size_t)i1 + 1];
-                for (uint32_t p = coff[row + (size_t)i0]; p < p1; p++) {
-                    const float4 b = spos[p];
-                    const double dx = (double)(b.x - a.x), dy = (double)(b.y - a.y), dz = (double)(b.z - a.z);
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    const double w = exp(-d2 / den);
-                    const double4 v = sv[p];
-                    tw += w;
-                    sx += w * v.x;
-                    sy += w * v.y;
-                    sz += w * v.z;
+                uint32_t p0 = __builtin_amdgcn_readfirstlane(coff[row + (size_t)X0]);
+                for (int ii = X0; ii <= X1; ii++) {
+                    const uint32_t p1 = __builtin_amdgcn_readfirstlane(coff[row + (size_t)ii + 1]);
+                    const bool cin = rin && abs(ii - i) <= 2;
+                    if (__ballot(cin))
+                        for (uint32_t p = p0; p < p1; p++) pair(p, cin);
+                    p0 = p1;
                 }
             }
+        }
+    } else {
+        const int i0 = max(i - 2, 0), i1 = min(i + 2, gi - 1);
+        if (i0 <= i1)
+            for (int kk = max(k - 2, 0); kk <= min(k + 2, gi - 1); kk++)
+                for (int jj = max(j - 2, 0); jj <= min(j + 2, gi - 1); jj++) {
+                    const size_t row = ((size_t)kk * g + (size_t)jj) * g;
+                    const uint32_t p1 = coff[row + (size_t)i1 + 1];
+                    for (uint32_t p = coff[row + (size_t)i0]; p < p1; p++) pair(p, true);
+                }
+    }
+    if (!live) return;
     if (tw < 1e-12) tw = 1e+24;
     const size_t o = (size_t)idx[t] * o_stride;
     ox[o] = sx / tw;
@@ -194,9 +266,23 @@ pppm_status launch_gaussian_smooth(const float4* spos, const uint32_t* coff, con
     hipLaunchKernelGGL(k_gather_values, dim3(div_up(n, 256)), dim3(256), 0, st, idx, vx, vy, vz, v_stride, n,
                        scratch);
     PPPM_LAUNCH_CHECK();
-    const double den = 2 * nu * nu;
-    hipLaunchKernelGGL(k_gaussian_smooth, dim3(div_up(n, 256)), dim3(256), 0, st, spos, scratch, coff, idx, n,
-                       pg.hash.g, pg.invh, den, ox, oy, oz, o_stride);
+    const double nid = -1.0 / (2 * nu * nu);
+    // degree of the Taylor weight (0: library exp), from the exponent bound
+    const double xm = 28.0 * pg.h * pg.h * -nid;
+    int deg = 0;
+    double fact = 720.0;  // (deg + 1)! for deg = 5, 7, 9, 11
+    for (int d = 5; d <= 11 && deg == 0; d += 2) {
+        if (xm <= 0.125 && std::pow(xm, d + 1) / fact <= 1e-17) deg = d;
+        fact *= (double)(d + 2) * (double)(d + 3);
+    }
+    const dim3 grid(div_up(n, 256)), blk(256);
+    switch (deg) {
+        case 5: hipLaunchKernelGGL(k_gaussian_smooth<5>, grid, blk, 0, st, spos, scratch, coff, idx, n, pg.hash.g, pg.invh, nid, ox, oy, oz, o_stride); break;
+        case 7: hipLaunchKernelGGL(k_gaussian_smooth<7>, grid, blk, 0, st, spos, scratch, coff, idx, n, pg.hash.g, pg.invh, nid, ox, oy, oz, o_stride); break;
+        case 9: hipLaunchKernelGGL(k_gaussian_smooth<9>, grid, blk, 0, st, spos, scratch, coff, idx, n, pg.hash.g, pg.invh, nid, ox, oy, oz, o_stride); break;
+        case 11: hipLaunchKernelGGL(k_gaussian_smooth<11>, grid, blk, 0, st, spos, scratch, coff, idx, n, pg.hash.g, pg.invh, nid, ox, oy, oz, o_stride); break;
+        default: hipLaunchKernelGGL(k_gaussian_smooth<0>, grid, blk, 0, st, spos, scratch, coff, idx, n, pg.hash.g, pg.invh, nid, ox, oy, oz, o_stride); break;
+    }
     PPPM_LAUNCH_CHECK();
     return PPPM_OK;
 }

@@ -136,6 +136,31 @@ def test_smooth_vorticity_context(native, oracle, g):
         np.testing.assert_array_equal(np.stack([q.to_array(np.float64) for q in d_i]), got)
 
 
+@pytest.mark.parametrize("g", [64, 128, 256])
+def test_smooth_vorticity_fine_grid(native, oracle, g):
+    """Grids fine enough (28 h^2 / (2 nu^2) <= 1/8) that the kernel takes
+    the Taylor-series weight in place of exp (degrees 11, 9, 7 here)."""
+    DB = native.DeviceBuffer
+    n = 30_000
+    rng = np.random.default_rng(23)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = (0.4 + 0.2 * rng.random((n, 3))).astype(np.float32)
+    v = rng.standard_normal((3, n))
+    o, L = oracle.reference_domain(pos, g)
+    h = L / g
+    assert 28.0 * h * h / (2 * 0.1 * 0.1) <= 0.125
+    print(f"[timestep] smooth_vorticity fine grid n={n} g={g} h={h:.4g}", flush=True)
+    ctx = native.Context(native.config_params(g, h=h, origin=o))
+    sc = native.Scene(pos, v)
+    ctx.set_vortices(sc.n, sc.pos.ptr, sc.w_ptrs())
+    d_o = [DB(8 * n) for _ in range(3)]
+    ctx.smooth_vorticity(sc.w_ptrs(), [q.ptr for q in d_o])
+    native.synchronize()
+    got = np.stack([q.to_array(np.float64) for q in d_o])
+    ref = oracle.gaussian_smooth(oracle.make_params(g, h, o), pos, v, 0.1)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
+
+
 def test_smooth_vorticity_errors(native):
     p = native.config_params(16)
     ctx = native.Context(p)
