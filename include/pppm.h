@@ -377,6 +377,30 @@ enum { PPPM_STEP_KEEP_DOMAIN = 1 };
 pppm_status pppm_time_step(pppm_ctx *ctx, uint32_t n, float *cent4, double *wx, double *wy, double *wz, double dt,
                            double seg_h, double nu, int32_t flags);
 
+/* ---- baroclinic source (addBaroclinicForce) ---------------------------------
+ * VortexSimulationSystemImp.cpp:369-465 with setDensityDomain (320-367) and
+ * the kernels AssignDensity, GradDensity, ParticleGetBaroclinic
+ * (GPU_kernel.cu:3655-3898).  The smoke particles (float4 positions, float
+ * values) get a density grid over 1.2 x their extent with dim =
+ * min(floor(L / cell_target) + 1, max_dim) cells per axis (the reference:
+ * cell_target 0.2, max_dim 200); the grid value is the (1 - d^2)^4-weighted
+ * mean of the particles within one cell (an empty cell counts its own centre
+ * with value 0); its periodic central differences are interpolated at each
+ * vortex with hat weights, and w += dt_beta * 0.1 * (grad rho x g) (the
+ * reference: dt_beta = 5 dt, g = (0, 10, 0)).  A density object keeps its
+ * buffers between calls.  Device pointers; enqueued on hip_stream, with one
+ * synchronisation for the smoke's bounding box.  nheat = 0 does nothing. */
+typedef struct pppm_density pppm_density;
+pppm_status pppm_density_create(pppm_density **out);
+void pppm_density_destroy(pppm_density *d);
+pppm_status pppm_add_baroclinic(pppm_density *d, const float *heat_pos4, const float *heat_rho, uint32_t nheat,
+                                const float *cent4, double *wx, double *wy, double *wz, uint32_t n, double dt_beta,
+                                const double gvec[3], double cell_target, uint32_t max_dim, void *hip_stream);
+/* The last call's grid: density (dim^3 floats), gradient (3 floats per cell),
+ * dim, origin and cell size; device pointers owned by the object. */
+pppm_status pppm_density_grid(const pppm_density *d, const float **rho, const float **grad3, uint32_t *dim,
+                              double origin[3], double *h);
+
 /* ---- timing on the library's streams --------------------------------------- */
 typedef struct pppm_timer pppm_timer;
 pppm_status pppm_timer_create(pppm_timer **out);

@@ -80,6 +80,18 @@ void AddURotation(double4 *U, double *u, double *v, double *w, uint32_t num);
 void ParticleGaussianSmooth(uint32_t *start, uint32_t *end, float4 *pos, float4 *pos_sort, double nu, double dt,
                             double *o_vortx, double *o_vorty, double *o_vortz, double *i_vortx, double *i_vorty,
                             double *i_vortz, double h, uint3 gdim, uint32_t N, double4 origin);
+/* Baroclinic source, ParticleMesh.h (AssignDensity, GradDensity,
+ * ParticleGetBaroclinic), GPU_kernel.cu:3655-3898: grid density from the
+ * sorted smoke over start/end tables (cubic grid, hash grid = density grid),
+ * its periodic central differences, and w += dt_beta * 0.1 * (grad rho x
+ * (0, 10, 0)) at each vortex.  vort_size, hashdim, dimy and dimz are unused
+ * (cubic grids), as in the reference. */
+void AssignDensity(uint32_t *start, uint32_t *end, float4 *pos, float *rho, float *vort_size, double cell_h,
+                   float *gridval, uint3 griddim, uint3 hashdim, uint32_t num_particle, double4 origin);
+void GradDensity(float *phi, float3 *dphi, double cell_h, uint32_t dimx, uint32_t dimy, uint32_t dimz);
+void ParticleGetBaroclinic(float4 *vortex_pos, double *vort_x, double *vort_y, double *vort_z, float3 *grad_rho,
+                           double dt_beta, uint32_t num_vort, uint32_t dimx, uint32_t dimy, uint32_t dimz,
+                           double inv_h, double4 origin);
 }
 
 #endif

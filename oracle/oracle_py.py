@@ -353,3 +353,35 @@ def time_step(cent4, omega, dt, seg_h, nu, g, K=2, kernel=KERNEL_EXPCUBIC, core_
     o2, L2 = reference_domain(c1, g) if domain is None else (o, L)
     w1 = gaussian_smooth(make_params(g, L2 / g, o2), c1, np.asarray(omega) + dw, nu)
     return c1, w1, {"ua": ua, "ub": ub, "domain": (o, L / g), "domain_after": (o2, L2 / g)}
+
+
+# ---- baroclinic source ----------------------------------------------------------
+def baroclinic(heat_pos4, heat_rho, cent4, omega, dt_beta, g=(0.0, 10.0, 0.0), cell_target=0.2, max_dim=200):
+    """addBaroclinicForce (VortexSimulationSystemImp.cpp:369-465): density
+    domain and grid, hash of the smoke on it, AssignDensity, GradDensity,
+    ParticleGetBaroclinic.  Returns (omega (3, N), stages dict)."""
+    heat_pos4 = _f32(heat_pos4)
+    nh = heat_pos4.shape[0]
+    o = (C.c_double * 3)()
+    h = C.c_double(0)
+    dim = C.c_uint32(0)
+    lib().orc_density_domain(_p(heat_pos4), C.c_uint32(nh), C.c_double(cell_target), C.c_uint32(max_dim), o,
+                             C.byref(h), C.byref(dim))
+    dim, h, origin = dim.value, h.value, [o[0], o[1], o[2]]
+    key, idx = hash_keys(heat_pos4, dim, float(np.float32(h)), [float(np.float32(v)) for v in origin])
+    key, idx = sort_pairs(key, idx)
+    start, end = cell_tables(key, dim ** 3)
+    spos = np.ascontiguousarray(heat_pos4[idx])
+    srho = np.ascontiguousarray(_f32(heat_rho)[idx])
+    grid = np.empty(dim ** 3, np.float32)
+    lib().orc_assign_density(_p(start), _p(end), _p(spos), _p(srho), C.c_double(h), C.c_uint32(dim),
+                             C.c_uint32(nh), o, _p(grid))
+    grad = np.empty(3 * dim ** 3, np.float32)
+    lib().orc_grad_density(_p(grid), C.c_uint32(dim), C.c_double(h), _p(grad))
+    cent4 = _f32(cent4)
+    w = [np.array(omega[c], np.float64) for c in range(3)]
+    gv = (C.c_double * 3)(*[float(v) for v in g])
+    lib().orc_baroclinic(_p(cent4), _p(w[0]), _p(w[1]), _p(w[2]), _p(grad), C.c_double(dt_beta),
+                         C.c_uint32(cent4.shape[0]), C.c_uint32(dim), C.c_double(1.0 / float(np.float32(h))), o, gv)
+    return np.stack(w), {"dim": dim, "h": h, "origin": origin, "rho": grid, "grad": grad.reshape(-1, 3),
+                         "start": start, "end": end, "spos": spos, "srho": srho}

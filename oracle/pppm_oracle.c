@@ -1274,3 +1274,149 @@ void orc_gaussian_smooth(const uint32_t *start, const uint32_t *end, const float
         oz[t] = sz / tw;
     }
 }
+
+/* ---- baroclinic source (addBaroclinicForce) ---------------------------
+ * VortexSimulationSystemImp.cpp:320-367 (setDensityDomain), 369-465, and
+ * GPU_kernel.cu:3655-3898. */
+
+/* setDensityDomain + the grid rule of addBaroclinicForce: L = 1.2 x the
+ * largest extent, origin = box centre - L/2, dim = min(floor(L/cell) + 1,
+ * max_dim), h = L / dim. */
+void orc_density_domain(const float *pos4, uint32_t n, double cell_target, uint32_t max_dim, double origin[3],
+                        double *h_out, uint32_t *dim_out)
+{
+    double mn[3], mx[3], L;
+    uint32_t i, dim;
+    int c;
+    for (c = 0; c < 3; c++) mn[c] = mx[c] = pos4[c];
+    for (i = 1; i < n; i++)
+        for (c = 0; c < 3; c++) {
+            double v = pos4[4 * (size_t)i + c];
+            if (v < mn[c]) mn[c] = v; else if (v > mx[c]) mx[c] = v;
+        }
+    L = mx[0] - mn[0];
+    if (mx[1] - mn[1] > L) L = mx[1] - mn[1];
+    if (mx[2] - mn[2] > L) L = mx[2] - mn[2];
+    L = 1.2 * L;
+    for (c = 0; c < 3; c++) origin[c] = (mn[c] + 0.5 * (mx[c] - mn[c])) - 0.5 * L;
+    dim = (uint32_t)floor(L / cell_target) + 1;
+    if (dim > max_dim) dim = max_dim;
+    *dim_out = dim;
+    *h_out = L / (double)dim;
+}
+
+static double orc_poly_shape2(double r)
+{
+    double rr = fabs(r), r1 = 1 - rr, r2 = r1 * r1;
+    return rr < 1 ? r2 * r2 : 0.0;
+}
+
+static double orc_poly_shape(double r)
+{
+    double rr = fabs(r);
+    return rr < 1 ? 1 - rr : 0.0;
+}
+
+/* GPU_kernel.cu:3655-3725 assign_density_kernel over the +-2 hash cells.  An
+ * empty cell (start = EMPTY) takes the `end < start` branch: the reference
+ * never clears its end table, so the branch adds the weight of the empty
+ * cell's centre with value 0. */
+void orc_assign_density(const uint32_t *start, const uint32_t *end, const float *pos4_sorted, const float *rho,
+                        double cell_h, uint32_t dim, uint32_t n, const double origin[3], float *grid)
+{
+    const double invh = 1.0 / cell_h;
+    const int d = (int)dim;
+    int64_t c;
+#pragma omp parallel for schedule(static)
+    for (c = 0; c < (int64_t)dim * dim * dim; c++) {
+        int i = (int)(c % dim), j = (int)((c / dim) % dim), k = (int)(c / ((int64_t)dim * dim));
+        double gx = (double)i + 0.5, gy = (double)j + 0.5, gz = (double)k + 0.5;
+        double sum = 0, tw = 0;
+        int kk, jj, ii;
+        for (kk = k - 2; kk <= k + 2; kk++)
+            for (jj = j - 2; jj <= j + 2; jj++)
+                for (ii = i - 2; ii <= i + 2; ii++) {
+                    uint32_t q, p;
+                    if (!(kk >= 0 && kk < d && jj >= 0 && jj < d && ii >= 0 && ii < d)) continue;
+                    q = ((uint32_t)kk * dim + (uint32_t)jj) * dim + (uint32_t)ii;
+                    if (start[q] != EMPTY && end[q] >= start[q]) {
+                        for (p = start[q]; p < end[q]; p++) {
+                            float px, py, pz;
+                            double dx, dy, dz, w;
+                            if (p >= n) continue;
+                            px = (float)((double)pos4_sorted[4 * (size_t)p] - origin[0]);
+                            py = (float)((double)pos4_sorted[4 * (size_t)p + 1] - origin[1]);
+                            pz = (float)((double)pos4_sorted[4 * (size_t)p + 2] - origin[2]);
+                            dx = gx - px * invh;
+                            dy = gy - py * invh;
+                            dz = gz - pz * invh;
+                            w = orc_poly_shape2(dx * dx + dy * dy + dz * dz);
+                            sum += w * (double)rho[p];
+                            tw += w;
+                        }
+                    } else {
+                        double dx = gx - ((double)ii + 0.5), dy = gy - ((double)jj + 0.5), dz = gz - ((double)kk + 0.5);
+                        double w = orc_poly_shape2(dx * dx + dy * dy + dz * dz);
+                        sum += 0.0;
+                        tw += w;
+                    }
+                }
+        if (tw < 1e-12) tw = 1e+24;
+        grid[c] = (float)(sum / tw);
+    }
+}
+
+/* GPU_kernel.cu:3752-3791 grad_density_kernel: periodic central differences. */
+void orc_grad_density(const float *phi, uint32_t dim, double cell_h, float *grad)
+{
+    const double invh = 1.0 / cell_h;
+    uint64_t c, G = (uint64_t)dim * dim * dim;
+    for (c = 0; c < G; c++) {
+        uint32_t i = (uint32_t)(c % dim), j = (uint32_t)((c / dim) % dim), k = (uint32_t)(c / ((uint64_t)dim * dim));
+        uint32_t im = (i + dim - 1) % dim, ip = (i + 1) % dim, jm = (j + dim - 1) % dim, jp = (j + 1) % dim;
+        uint32_t km = (k + dim - 1) % dim, kp = (k + 1) % dim;
+#define PHI(a, b, e) ((double)phi[((uint64_t)(e) * dim + (b)) * dim + (a)])
+        grad[3 * c] = (float)((PHI(ip, j, k) - PHI(im, j, k)) * 0.5 * invh);
+        grad[3 * c + 1] = (float)((PHI(i, jp, k) - PHI(i, jm, k)) * 0.5 * invh);
+        grad[3 * c + 2] = (float)((PHI(i, j, kp) - PHI(i, j, km)) * 0.5 * invh);
+#undef PHI
+    }
+}
+
+/* GPU_kernel.cu:3810-3880 baroclinic_kernel over the +-2 grid cells:
+ * w += dt_beta * 0.1 * (sum_cells grad hat-weight) x g. */
+void orc_baroclinic(const float *vpos4, double *wx, double *wy, double *wz, const float *grad, double dt_beta,
+                    uint32_t n, uint32_t dim, double inv_h, const double origin[3], const double g[3])
+{
+    const int d = (int)dim;
+    int64_t t;
+#pragma omp parallel for schedule(static)
+    for (t = 0; t < (int64_t)n; t++) {
+        float px = (float)((double)vpos4[4 * t] - origin[0]);
+        float py = (float)((double)vpos4[4 * t + 1] - origin[1]);
+        float pz = (float)((double)vpos4[4 * t + 2] - origin[2]);
+        double qx = px * inv_h, qy = py * inv_h, qz = pz * inv_h;
+        int i = (int)floor(qx), j = (int)floor(qy), k = (int)floor(qz);
+        double sx = 0, sy = 0, sz = 0, bx, by, bz;
+        int kk, jj, ii;
+        for (kk = k - 2; kk <= k + 2; kk++)
+            for (jj = j - 2; jj <= j + 2; jj++)
+                for (ii = i - 2; ii <= i + 2; ii++) {
+                    uint64_t q;
+                    double w;
+                    if (!(kk >= 0 && kk < d && jj >= 0 && jj < d && ii >= 0 && ii < d)) continue;
+                    q = ((uint64_t)kk * dim + (uint64_t)jj) * dim + (uint64_t)ii;
+                    w = orc_poly_shape((double)ii + 0.5 - qx) * orc_poly_shape((double)jj + 0.5 - qy) *
+                        orc_poly_shape((double)kk + 0.5 - qz);
+                    sx += (double)grad[3 * q] * w;
+                    sy += (double)grad[3 * q + 1] * w;
+                    sz += (double)grad[3 * q + 2] * w;
+                }
+        bx = sy * g[2] - sz * g[1];
+        by = sz * g[0] - sx * g[2];
+        bz = sx * g[1] - sy * g[0];
+        wx[t] += dt_beta * 0.1 * bx;
+        wy[t] += dt_beta * 0.1 * by;
+        wz[t] += dt_beta * 0.1 * bz;
+    }
+}

@@ -123,6 +123,15 @@ void orc_gaussian_smooth(const uint32_t *start, const uint32_t *end, const float
                          const double *ix, const double *iy, const double *iz, double h, int g, uint32_t n,
                          const double origin[3]);
 
+/* ---- baroclinic source (addBaroclinicForce) ------------------------- */
+void orc_density_domain(const float *pos4, uint32_t n, double cell_target, uint32_t max_dim, double origin[3],
+                        double *h_out, uint32_t *dim_out);
+void orc_assign_density(const uint32_t *start, const uint32_t *end, const float *pos4_sorted, const float *rho,
+                        double cell_h, uint32_t dim, uint32_t n, const double origin[3], float *grid);
+void orc_grad_density(const float *phi, uint32_t dim, double cell_h, float *grad);
+void orc_baroclinic(const float *vpos4, double *wx, double *wy, double *wz, const float *grad, double dt_beta,
+                    uint32_t n, uint32_t dim, double inv_h, const double origin[3], const double g[3]);
+
 int  orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -57,6 +57,7 @@ def lib():
         L.pppm_destroy.restype = None
         L.pppm_hash_destroy.restype = None
         L.pppm_mg_destroy.restype = None
+        L.pppm_density_destroy.restype = None
         L.pppm_timer_destroy.restype = None
         L.pppm_params_config_preset.restype = None
         L.pppm_params_reference_preset.restype = None
@@ -189,6 +190,46 @@ def vector_add(A, B, Cp, a, b, n, stream=None):
 def add_velocity(U4, u3, n, stream=None):
     """pppm_add_velocity: U.xyz += u3."""
     check(lib().pppm_add_velocity(_vp(U4), *[_vp(q) for q in u3], C.c_uint32(n), _vp(stream)), "add_velocity")
+
+
+class Density:
+    """pppm_density: the baroclinic source (addBaroclinicForce)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().pppm_density_create(C.byref(self.h)), "density_create")
+
+    def add_baroclinic(self, heat_pos, heat_rho, nheat, cent, w3, n, dt_beta, g=(0.0, 10.0, 0.0),
+                       cell_target=0.2, max_dim=200, stream=None):
+        gv = (C.c_double * 3)(*[float(v) for v in g])
+        check(lib().pppm_add_baroclinic(self.h, _vp(heat_pos), _vp(heat_rho), C.c_uint32(nheat), _vp(cent),
+                                        *[_vp(q) for q in w3], C.c_uint32(n), C.c_double(dt_beta), gv,
+                                        C.c_double(cell_target), C.c_uint32(max_dim), _vp(stream)),
+              "add_baroclinic")
+
+    def grid(self):
+        """(rho (dim^3,) float32, grad (dim^3, 3) float32, dim, origin, h), host copies."""
+        rho, grad = C.c_void_p(), C.c_void_p()
+        dim = C.c_uint32(0)
+        o = (C.c_double * 3)()
+        h = C.c_double(0)
+        check(lib().pppm_density_grid(self.h, C.byref(rho), C.byref(grad), C.byref(dim), o, C.byref(h)),
+              "density_grid")
+        G = dim.value ** 3
+        return (dev_to_array(rho.value, np.float32, G), dev_to_array(grad.value, np.float32, 3 * G).reshape(G, 3),
+                dim.value,
+                [o[0], o[1], o[2]], h.value)
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().pppm_density_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_frame(path):
