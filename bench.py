@@ -218,29 +218,43 @@ def cpu_baseline():
     return out
 
 
-def run_time_step(N, pos, w, g, u_ref, steps):
-    """solveTimeStep on device state (pppm_time_step, mixed precision, the
-    reference's seg_h = nu = 0.1) on the workload's fixed grid ([0,1)^3,
-    h = 1/g), dt = 0.5 h / max|u| from a velocity evaluation of the same
-    state.  One untimed step first (allocation)."""
+def run_time_step(N, pos, w, rho, g, u_ref, steps):
+    """solveTimeStep on device state: addBaroclinicForce (pppm_add_baroclinic,
+    smoke = the vortices with their U(0,1) density, the reference's density
+    grid rule, dt_beta = 5 dt, g = (0, 0, -1)) then pppm_time_step (mixed
+    precision, the reference's seg_h = nu = 0.1) on the workload's fixed grid
+    ([0,1)^3, h = 1/g); dt = 0.5 h / max|u| from a velocity evaluation of the
+    same state.  One untimed step first (allocation)."""
     n = pos.shape[0]
     p = N.config_params(g, precision=N.PREC_MIXED, mg_tol=MIXED_TOL, mg_max_cycles=40)
     ctx = N.Context(p)
+    dens = N.Density() if rho is not None else None
     dt = 0.5 * (1.0 / g) / float(np.abs(u_ref).max())
     cent = N.DeviceBuffer.from_array(np.ascontiguousarray(pos, np.float32))
     wd = [N.DeviceBuffer.from_array(np.ascontiguousarray(w[c], np.float64)) for c in range(3)]
-    ctx.time_step(n, cent.ptr, [b.ptr for b in wd], dt, 0.1, 0.1, keep_domain=True)
-    N.synchronize()
-    lat = []
-    for _ in range(steps):
-        s0 = time.perf_counter()
+    smoke = N.DeviceBuffer.from_array(np.ascontiguousarray(rho, np.float32)) if dens else None
+
+    def step():
+        t0 = time.perf_counter()
+        if dens:
+            dens.add_baroclinic(cent.ptr, smoke.ptr, n, cent.ptr, [b.ptr for b in wd], n, 5 * dt, (0.0, 0.0, -1.0),
+                                0.2, 200, ctx.stream)
+            N.synchronize()
+        t1 = time.perf_counter()
         ctx.time_step(n, cent.ptr, [b.ptr for b in wd], dt, 0.1, 0.1, keep_domain=True)
         N.synchronize()
-        lat.append(time.perf_counter() - s0)
+        return t1 - t0, time.perf_counter() - t1
+
+    step()
+    lat = [step() for _ in range(steps)]
     ctx.close()
-    ms = 1e3 * float(np.mean(lat))
-    return {"ms_per_step": ms, "vortex_steps_per_s": n / (ms * 1e-3), "steps": steps, "dt": dt, "seg_h": 0.1,
-            "nu": 0.1, "precision": "mixed", "domain": "fixed [0,1)^3, h = 1/g",
+    ms = 1e3 * float(np.mean([a + b for a, b in lat]))
+    return {"ms_per_step": ms, "vortex_steps_per_s": n / (ms * 1e-3), "steps": steps, "dt": dt,
+            "baroclinic_ms": 1e3 * float(np.mean([a for a, _ in lat])),
+            "segment_step_ms": 1e3 * float(np.mean([b for _, b in lat])),
+            "seg_h": 0.1, "nu": 0.1, "precision": "mixed", "domain": "fixed [0,1)^3, h = 1/g",
+            "baroclinic": "density grid by the reference rule (cells of 0.2, at most 200), g = (0,0,-1)"
+                          if dens else None,
             "integrator": "forward Euler of segment ends (solveTimeStep)"}
 
 
@@ -256,7 +270,7 @@ def main():
     if N.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(3)
-    pos, w, g = scenes.make(a.workload)
+    pos, w, g, smoke = scenes.make_with_smoke(a.workload)
     n = pos.shape[0]
     log(f"[bench] workload {a.workload}: {n} vortices, {g}^3 grid")
     scene = N.Scene(pos, w.astype(np.float32), N.F32)
@@ -325,7 +339,7 @@ def main():
     if a.time_steps > 0:
         u_ref = (res.get("mixed") or res.get("fp32") or res.get("served"))["u_last"]
         log(f"[bench] time step: {a.time_steps} steps of pppm_time_step")
-        out["time_step"] = run_time_step(N, pos, w, g, u_ref, a.time_steps)
+        out["time_step"] = run_time_step(N, pos, w, smoke, g, u_ref, a.time_steps)
         log(f"[bench]   {out['time_step']['ms_per_step']:.1f} ms/step")
     if a.copy_calibration:
         rows = 16 << 20

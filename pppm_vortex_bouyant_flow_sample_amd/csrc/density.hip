@@ -93,6 +93,66 @@ __global__ void __launch_bounds__(256) k_assign_density(const uint32_t* __restri
     grid[c] = (float)(sum / tw);
 }
 
+// The same grid value with one workgroup per grid cell, for grids whose
+// cells hold many smoke particles (the reference's rule gives 7^3 cells for a
+// unit-sized cloud: a thread per cell would walk a seventh of the smoke
+// alone).  The threads stride over the sorted smoke of each of the 27
+// cells, and a fixed-order tree in LDS adds their partial sums, so the
+// result is deterministic and within one float rounding of the sequential
+// sum.
+constexpr int kDensityBlock = 256;
+
+__global__ void __launch_bounds__(kDensityBlock) k_assign_density_block(
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const float4* __restrict__ pos,
+    const float* __restrict__ rho, float* __restrict__ grid, uint32_t dim, double invh, uint32_t n, double ox,
+    double oy, double oz) {
+    __shared__ double red[2][kDensityBlock];
+    const uint32_t c = blockIdx.x;
+    const int d = (int)dim;
+    const int i = (int)(c % dim), j = (int)((c / dim) % dim), k = (int)(c / (dim * dim));
+    const double gx = (double)i + 0.5, gy = (double)j + 0.5, gz = (double)k + 0.5;
+    double sum = 0, tw = 0;
+    bool own_empty = false;
+    for (int kk = k - 1; kk <= k + 1; kk++)
+        for (int jj = j - 1; jj <= j + 1; jj++) {
+            if (kk < 0 || kk >= d || jj < 0 || jj >= d) continue;
+            const uint32_t row = ((uint32_t)kk * dim + (uint32_t)jj) * dim;
+            for (int ii = i - 1; ii <= i + 1; ii++) {
+                if (ii < 0 || ii >= d) continue;
+                const uint32_t q = row + (uint32_t)ii;
+                const uint32_t s0 = start[q];
+                if (s0 == 0xFFFFFFFFu) {
+                    if (q == c) own_empty = true;
+                    continue;
+                }
+                const uint32_t e0 = min(end[q], n);
+                for (uint32_t p = s0 + threadIdx.x; p < e0; p += kDensityBlock) {
+                    const float4 a = pos[p];
+                    const float px = (float)((double)a.x - ox), py = (float)((double)a.y - oy),
+                                pz = (float)((double)a.z - oz);
+                    const double w = density_weight(gx, gy, gz, px * invh, py * invh, pz * invh);
+                    sum += w * (double)rho[p];
+                    tw += w;
+                }
+            }
+        }
+    red[0][threadIdx.x] = sum;
+    red[1][threadIdx.x] = tw;
+    __syncthreads();
+    for (int s = kDensityBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double t = red[1][0] + (own_empty ? 1.0 : 0.0);
+        if (t < 1e-12) t = 1e+24;
+        grid[c] = (float)(red[0][0] / t);
+    }
+}
+
 // grad_density_kernel (GPU_kernel.cu:3752-3791): periodic central
 // differences, (r - l) * 0.5 * invh per axis, stored as float.
 __global__ void __launch_bounds__(256) k_grad_density(const float* __restrict__ phi, float* __restrict__ grad,
@@ -144,6 +204,9 @@ __global__ void __launch_bounds__(256) k_baroclinic(const float4* __restrict__ v
 }
 
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+// Smoke particles per grid cell from which a workgroup takes each cell.
+constexpr uint64_t kDensityBlockMin = 64;
 
 template <typename T>
 pppm_status grow(T** p, size_t* cap, size_t count) {
@@ -266,8 +329,13 @@ extern "C" pppm_status pppm_add_baroclinic(pppm_density* d, const float* heat_po
     TRY(launch_gather(vs, heat_pos4, d->spos, 4, PPPM_F32, nheat, st));
     TRY(launch_gather(vs, heat_rho, d->srho, 1, PPPM_F32, nheat, st));
     // AssignDensity, GradDensity, ParticleGetBaroclinic
-    hipLaunchKernelGGL(k_assign_density, dim3(blocks_for(G)), dim3(256), 0, st, d->start, d->end, d->spos, d->srho,
-                       d->rho, dim, 1.0 / h, nheat, d->origin[0], d->origin[1], d->origin[2]);
+    if ((uint64_t)nheat >= kDensityBlockMin * (uint64_t)G)
+        hipLaunchKernelGGL(k_assign_density_block, dim3((uint32_t)G), dim3(kDensityBlock), 0, st, d->start, d->end,
+                           d->spos, d->srho, d->rho, dim, 1.0 / h, nheat, d->origin[0], d->origin[1],
+                           d->origin[2]);
+    else
+        hipLaunchKernelGGL(k_assign_density, dim3(blocks_for(G)), dim3(256), 0, st, d->start, d->end, d->spos,
+                           d->srho, d->rho, dim, 1.0 / h, nheat, d->origin[0], d->origin[1], d->origin[2]);
     PPPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_grad_density, dim3(blocks_for(G)), dim3(256), 0, st, d->rho, d->grad, dim, 1.0 / h);
     PPPM_LAUNCH_CHECK();

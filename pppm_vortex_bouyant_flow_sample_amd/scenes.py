@@ -88,3 +88,12 @@ def make(name, n=None, seed=None):
     n = n0 if n is None else n
     out = gen(n) if seed is None else gen(n, seed=seed)
     return out[0], out[1], g
+
+
+def make_with_smoke(name, n=None, seed=None):
+    """make() plus the per-particle smoke density the generator draws (the
+    plume's U(0,1)), or None for scenes without smoke."""
+    gen, n0, g = WORKLOADS[name]
+    n = n0 if n is None else n
+    out = gen(n) if seed is None else gen(n, seed=seed)
+    return out[0], out[1], g, (out[2] if len(out) > 2 else None)
