@@ -218,13 +218,15 @@ def cpu_baseline():
     return out
 
 
-def run_time_step(N, pos, w, rho, g, u_ref, steps):
+def run_time_step(N, pos, w, rho, g, u_ref, steps, rk2):
     """solveTimeStep on device state: addBaroclinicForce (pppm_add_baroclinic,
     smoke = the vortices with their U(0,1) density, the reference's density
     grid rule, dt_beta = 5 dt, g = (0, 0, -1)) then pppm_time_step (mixed
     precision, the reference's seg_h = nu = 0.1) on the workload's fixed grid
     ([0,1)^3, h = 1/g); dt = 0.5 h / max|u| from a velocity evaluation of the
-    same state.  One untimed step first (allocation)."""
+    same state.  rk2: Heun's method on the segment ends (the long-run
+    configuration's integrator) instead of the reference's forward Euler.
+    One untimed step first (allocation)."""
     n = pos.shape[0]
     p = N.config_params(g, precision=N.PREC_MIXED, mg_tol=MIXED_TOL, mg_max_cycles=40)
     ctx = N.Context(p)
@@ -241,7 +243,7 @@ def run_time_step(N, pos, w, rho, g, u_ref, steps):
                                 0.2, 200, ctx.stream)
             N.synchronize()
         t1 = time.perf_counter()
-        ctx.time_step(n, cent.ptr, [b.ptr for b in wd], dt, 0.1, 0.1, keep_domain=True)
+        ctx.time_step(n, cent.ptr, [b.ptr for b in wd], dt, 0.1, 0.1, keep_domain=True, rk2=rk2)
         N.synchronize()
         return t1 - t0, time.perf_counter() - t1
 
@@ -255,7 +257,7 @@ def run_time_step(N, pos, w, rho, g, u_ref, steps):
             "seg_h": 0.1, "nu": 0.1, "precision": "mixed", "domain": "fixed [0,1)^3, h = 1/g",
             "baroclinic": "density grid by the reference rule (cells of 0.2, at most 200), g = (0,0,-1)"
                           if dens else None,
-            "integrator": "forward Euler of segment ends (solveTimeStep)"}
+            "integrator": "RK2 (Heun) of segment ends" if rk2 else "forward Euler of segment ends (solveTimeStep)"}
 
 
 def main():
@@ -338,9 +340,10 @@ def main():
         out["vs_plain_fp32"] = value / out["fp32_path"]["value"] if head != "fp32" else 1.0
     if a.time_steps > 0:
         u_ref = (res.get("mixed") or res.get("fp32") or res.get("served"))["u_last"]
-        log(f"[bench] time step: {a.time_steps} steps of pppm_time_step")
-        out["time_step"] = run_time_step(N, pos, w, smoke, g, u_ref, a.time_steps)
-        log(f"[bench]   {out['time_step']['ms_per_step']:.1f} ms/step")
+        for key, rk2 in (("time_step", True), ("time_step_euler", False)):
+            log(f"[bench] time step ({'RK2' if rk2 else 'forward Euler'}): {a.time_steps} steps")
+            out[key] = run_time_step(N, pos, w, smoke, g, u_ref, a.time_steps, rk2)
+            log(f"[bench]   {out[key]['ms_per_step']:.1f} ms/step (baroclinic {out[key]['baroclinic_ms']:.1f})")
     if a.copy_calibration:
         rows = 16 << 20
         idx = N.DeviceBuffer.from_array(np.arange(rows, dtype=np.uint32))

@@ -372,8 +372,13 @@ pppm_status pppm_domain_device(pppm_ctx *ctx, const float *pos4, uint32_t n, dou
  * hashing (velocities, smoothing) it is set from the centres by
  * pppm_domain_device.  flags PPPM_STEP_KEEP_DOMAIN keeps the context's
  * domain instead (a fixed grid, as a long run on a given grid uses; vortices
- * outside it are handled as by pppm_set_vortices). */
-enum { PPPM_STEP_KEEP_DOMAIN = 1 };
+ * outside it are handled as by pppm_set_vortices).  PPPM_STEP_RK2 replaces
+ * the reference's forward Euler of the ends by Heun's method: the ends are
+ * predicted with the first velocities, the predicted segments give centres
+ * and strengths w + dw, the velocities at the predicted ends are taken from
+ * those vortices, and the ends move with the mean of the two velocities
+ * (two velocity evaluations per step; 96 more bytes per vortex). */
+enum { PPPM_STEP_KEEP_DOMAIN = 1, PPPM_STEP_RK2 = 2 };
 pppm_status pppm_time_step(pppm_ctx *ctx, uint32_t n, float *cent4, double *wx, double *wy, double *wz, double dt,
                            double seg_h, double nu, int32_t flags);
 

@@ -328,12 +328,14 @@ def gaussian_smooth(params, pos4, values, nu=0.1):
 
 
 def time_step(cent4, omega, dt, seg_h, nu, g, K=2, kernel=KERNEL_EXPCUBIC, core_a=None, tol=1e-12,
-              max_cycles=80, domain=None):
+              max_cycles=80, domain=None, rk2=False):
     """solveTimeStep without boundary, heat or density
     (VortexSimulationSystemImp.cpp:90-125): segment ends, the end velocities
     by the PPPM path on the centres' domain (own vortex excluded), forward
     Euler of the ends, new centres, vorticity change, Gaussian smoothing on the
     new centres' domain.  domain=(origin, h) keeps that fixed domain instead.
+    rk2: Heun's method on the ends (predicted ends, their segments' centres
+    and strengths omega + dw, velocities there, the mean of the two).
     Returns (centres (N, 4), omega (3, N), stages dict)."""
     cent4 = _f32(cent4)
     n = cent4.shape[0]
@@ -347,6 +349,19 @@ def time_step(cent4, omega, dt, seg_h, nu, g, K=2, kernel=KERNEL_EXPCUBIC, core_
     ua, _ = pppm_eval(p, cent4, omega, eval_pos4=enda, other_end4=cent4)
     u4a, u4b = np.zeros((n, 4)), np.zeros((n, 4))
     u4a[:, :3], u4b[:, :3] = ua.T, ub.T
+    if rk2:
+        a2, b2 = move_points(enda, dt, u4a), move_points(endb, dt, u4b)
+        c2 = segment_centres(a2, b2, cent4)
+        w2 = np.asarray(omega) + vortices_from_segments(a2, b2, omega, kappa)
+        if domain is None:
+            o2, L2 = reference_domain(c2, g)
+        else:
+            o2, L2 = o, L
+        p2 = make_params(g, L2 / g, o2, K=K, kernel=kernel, core_a=core_a, tol=tol, max_cycles=max_cycles)
+        ub2, _ = pppm_eval(p2, c2, w2, eval_pos4=b2, other_end4=c2)
+        ua2, _ = pppm_eval(p2, c2, w2, eval_pos4=a2, other_end4=c2)
+        u4a[:, :3] = 0.5 * ua.T + 0.5 * ua2.T
+        u4b[:, :3] = 0.5 * ub.T + 0.5 * ub2.T
     a1, b1 = move_points(enda, dt, u4a), move_points(endb, dt, u4b)
     c1 = segment_centres(a1, b1, cent4)
     dw = vortices_from_segments(a1, b1, omega, kappa)

@@ -203,6 +203,11 @@ struct pppm_ctx {
     double *kappa = nullptr, *dvort = nullptr;
     double4 *vel_a = nullptr, *vel_b = nullptr;
     uint32_t capStep = 0;
+    // RK2's predicted ends, centres, strengths and velocities
+    float4 *seg_a2 = nullptr, *seg_b2 = nullptr, *cent2 = nullptr;
+    double4 *vel_a2 = nullptr, *vel_b2 = nullptr;
+    double* vort2 = nullptr;
+    uint32_t capStep2 = 0;
     float* bbox = nullptr;
     uint32_t *cstart = nullptr, *cend = nullptr, *coff = nullptr, *scan_scr = nullptr;
     uint32_t* d_strays = nullptr;
@@ -320,6 +325,12 @@ struct pppm_ctx {
         dev_free(dvort);
         dev_free(vel_a);
         dev_free(vel_b);
+        dev_free(seg_a2);
+        dev_free(seg_b2);
+        dev_free(cent2);
+        dev_free(vel_a2);
+        dev_free(vel_b2);
+        dev_free(vort2);
         dev_free(bbox);
         dev_free(cstart);
         dev_free(cend);
@@ -877,8 +888,9 @@ extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, dou
     if (n == 0) return PPPM_OK;
     if (!cent4 || !wx || !wy || !wz) FAIL(PPPM_ERR_INVALID, "null array");
     if (!(seg_h > 0) || !(nu > 0)) FAIL(PPPM_ERR_INVALID, "seg_h and nu must be > 0");
-    if (flags & ~PPPM_STEP_KEEP_DOMAIN) FAIL(PPPM_ERR_INVALID, "unknown step flags");
+    if (flags & ~(PPPM_STEP_KEEP_DOMAIN | PPPM_STEP_RK2)) FAIL(PPPM_ERR_INVALID, "unknown step flags");
     const bool move_domain = !(flags & PPPM_STEP_KEEP_DOMAIN);
+    const bool rk2 = (flags & PPPM_STEP_RK2) != 0;
     if (n > c->capStep) {
         for (float4** q : {&c->seg_a, &c->seg_b}) dev_free(*q);
         dev_free(c->kappa);
@@ -894,6 +906,16 @@ extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, dou
         TRY(dev_alloc(&c->vel_b, n));
         c->capStep = n;
     }
+    if (rk2 && n > c->capStep2) {
+        for (float4** q : {&c->seg_a2, &c->seg_b2, &c->cent2}) dev_free(*q);
+        for (double4** q : {&c->vel_a2, &c->vel_b2}) dev_free(*q);
+        dev_free(c->vort2);
+        c->capStep2 = 0;
+        for (float4** q : {&c->seg_a2, &c->seg_b2, &c->cent2}) TRY(dev_alloc(q, n));
+        for (double4** q : {&c->vel_a2, &c->vel_b2}) TRY(dev_alloc(q, n));
+        TRY(dev_alloc(&c->vort2, 3 * (size_t)n));
+        c->capStep2 = n;
+    }
     hipStream_t st = c->st;
     float* a = (float*)c->seg_a;
     float* b = (float*)c->seg_b;
@@ -901,25 +923,56 @@ extern "C" pppm_status pppm_time_step(pppm_ctx* c, uint32_t n, float* cent4, dou
     double* ub = (double*)c->vel_b;
     double* dw[3] = {c->dvort, c->dvort + n, c->dvort + 2 * (size_t)n};
     double* w[3] = {wx, wy, wz};
-    double o[3], h;
+    // computeURotation: the vortices (centres cc, strengths ww) hashed on
+    // their own domain, the far field, then both ends in segment mode (the
+    // own vortex excluded), into double4 velocities
+    auto end_velocities = [&](const float* cc, double* const* ww, const float* ea, const float* eb, double* va,
+                              double* vb) -> pppm_status {
+        if (move_domain) {
+            double o[3], h;
+            TRY(pppm_domain_device(c, cc, n, o, &h));
+            TRY(pppm_set_domain(c, o, h));
+        }
+        TRY(pppm_set_vortices(c, n, cc, ww[0], ww[1], ww[2], 1, PPPM_F64));
+        TRY(pppm_compute_far_field(c));
+        TRY(pppm_evaluate(c, n, eb, cc, vb, vb + 1, vb + 2, 4, PPPM_F64));
+        return pppm_evaluate(c, n, ea, cc, va, va + 1, va + 2, 4, PPPM_F64);
+    };
     // convertVortToSeg
     TRY(pppm_segments_from_vortices(cent4, a, b, wx, wy, wz, c->kappa, seg_h, n, st));
-    // computeURotation: the centres hashed on their own domain, the far
-    // field, then both ends in segment mode (the own vortex excluded)
-    if (move_domain) {
-        TRY(pppm_domain_device(c, cent4, n, o, &h));
-        TRY(pppm_set_domain(c, o, h));
+    TRY(end_velocities(cent4, w, a, b, ua, ub));
+    if (rk2) {
+        // Heun: ends predicted with the first velocities, the predicted
+        // segments' centres and strengths w + dw (computeStretching's sum),
+        // the velocities there, then both ends move with the mean of the two
+        float* a2 = (float*)c->seg_a2;
+        float* b2 = (float*)c->seg_b2;
+        float* c2 = (float*)c->cent2;
+        double* w1[3] = {c->vort2, c->vort2 + n, c->vort2 + 2 * (size_t)n};
+        PPPM_HIP_TRY(hipMemcpyAsync(a2, a, 16 * (size_t)n, hipMemcpyDeviceToDevice, st));
+        PPPM_HIP_TRY(hipMemcpyAsync(b2, b, 16 * (size_t)n, hipMemcpyDeviceToDevice, st));
+        PPPM_HIP_TRY(hipMemcpyAsync(c2, cent4, 16 * (size_t)n, hipMemcpyDeviceToDevice, st));
+        TRY(pppm_apply_flow(a2, b2, c2, ua, ub, dt, n, st));
+        TRY(pppm_vortices_from_segments(a2, b2, wx, wy, wz, dw[0], dw[1], dw[2], c->kappa, n, st));
+        for (int q = 0; q < 3; q++) TRY(pppm_vector_add(w[q], dw[q], w1[q], 1.0, 1.0, n, st));
+        double* ua2 = (double*)c->vel_a2;
+        double* ub2 = (double*)c->vel_b2;
+        TRY(end_velocities(c2, w1, a2, b2, ua2, ub2));
+        // 4 doubles per velocity, in pieces that fit the 32-bit count
+        const size_t n4 = 4 * (size_t)n, piece = (size_t)1 << 30;
+        for (size_t o = 0; o < n4; o += piece) {
+            const uint32_t m = (uint32_t)std::min(piece, n4 - o);
+            TRY(pppm_vector_add(ua + o, ua2 + o, ua + o, 0.5, 0.5, m, st));
+            TRY(pppm_vector_add(ub + o, ub2 + o, ub + o, 0.5, 0.5, m, st));
+        }
     }
-    TRY(pppm_set_vortices(c, n, cent4, wx, wy, wz, 1, PPPM_F64));
-    TRY(pppm_compute_far_field(c));
-    TRY(pppm_evaluate(c, n, b, cent4, ub, ub + 1, ub + 2, 4, PPPM_F64));
-    TRY(pppm_evaluate(c, n, a, cent4, ua, ua + 1, ua + 2, 4, PPPM_F64));
     // applyFlow, convertSegToVort
     TRY(pppm_apply_flow(a, b, cent4, ua, ub, dt, n, st));
     TRY(pppm_vortices_from_segments(a, b, wx, wy, wz, dw[0], dw[1], dw[2], c->kappa, n, st));
     // computeStretching: w + dw smoothed over the new centres' neighbourhoods
     for (int q = 0; q < 3; q++) TRY(pppm_vector_add(w[q], dw[q], dw[q], 1.0, 1.0, n, st));
     if (move_domain) {
+        double o[3], h;
         TRY(pppm_domain_device(c, cent4, n, o, &h));
         TRY(pppm_set_domain(c, o, h));
     }

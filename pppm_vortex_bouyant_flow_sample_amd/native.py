@@ -319,11 +319,11 @@ class Context:
         check(lib().pppm_domain_device(self.h, _vp(pos_ptr), C.c_uint32(n), o, C.byref(h)), "domain_device")
         return [o[0], o[1], o[2]], h.value
 
-    def time_step(self, n, cent_ptr, w3, dt, seg_h=0.1, nu=0.1, keep_domain=False):
+    def time_step(self, n, cent_ptr, w3, dt, seg_h=0.1, nu=0.1, keep_domain=False, rk2=False):
         """pppm_time_step: one solveTimeStep on device state (in place)."""
+        flags = (1 if keep_domain else 0) | (2 if rk2 else 0)
         check(lib().pppm_time_step(self.h, C.c_uint32(n), _vp(cent_ptr), *[_vp(q) for q in w3], C.c_double(dt),
-                                   C.c_double(seg_h), C.c_double(nu), C.c_int32(1 if keep_domain else 0)),
-              "time_step")
+                                   C.c_double(seg_h), C.c_double(nu), C.c_int32(flags)), "time_step")
 
     def stats(self):
         s = Stats()
