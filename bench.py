@@ -248,11 +248,16 @@ def run_time_step(N, pos, w, rho, g, u_ref, steps, rk2):
         return t1 - t0, time.perf_counter() - t1
 
     step()
+    circ0 = [float(b.to_array(np.float64).sum()) for b in wd]
     lat = [step() for _ in range(steps)]
+    circ1 = [float(b.to_array(np.float64).sum()) for b in wd]
     ctx.close()
     ms = 1e3 * float(np.mean([a + b for a, b in lat]))
     return {"ms_per_step": ms, "vortex_steps_per_s": n / (ms * 1e-3), "steps": steps, "dt": dt,
             "baroclinic_ms": 1e3 * float(np.mean([a for a, _ in lat])),
+            "circulation": {"before": circ0, "after": circ1, "steps": steps,
+                            "rel_change": float(np.linalg.norm(np.subtract(circ1, circ0)) /
+                                                max(np.linalg.norm(circ0), 1e-300))},
             "segment_step_ms": 1e3 * float(np.mean([b for _, b in lat])),
             "seg_h": 0.1, "nu": 0.1, "precision": "mixed", "domain": "fixed [0,1)^3, h = 1/g",
             "baroclinic": "density grid by the reference rule (cells of 0.2, at most 200), g = (0,0,-1)"

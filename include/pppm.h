@@ -394,7 +394,9 @@ pppm_status pppm_time_step(pppm_ctx *ctx, uint32_t n, float *cent4, double *wx, 
  * vortex with hat weights, and w += dt_beta * 0.1 * (grad rho x g) (the
  * reference: dt_beta = 5 dt, g = (0, 10, 0)).  A density object keeps its
  * buffers between calls.  Device pointers; enqueued on hip_stream, with one
- * synchronisation for the smoke's bounding box.  nheat = 0 does nothing. */
+ * synchronisation for the smoke's bounding box.  nheat = 0 does nothing.
+ * Before pppm_time_step on the same strengths, pass the context's stream
+ * (pppm_get_stream) or synchronise: the two are not ordered otherwise. */
 typedef struct pppm_density pppm_density;
 pppm_status pppm_density_create(pppm_density **out);
 void pppm_density_destroy(pppm_density *d);

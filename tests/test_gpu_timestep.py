@@ -345,3 +345,45 @@ def test_time_step_rk2_against_oracle(native, oracle, keep):
     c_fe, _, _ = oracle.time_step(cent, w, dt, 0.1, 0.1, g, K=p.K, kernel=p.kernel, core_a=p.core_a, tol=1e-12,
                                   domain=(o, h) if keep else None)
     assert np.abs(c_fe[:, :3] - c_ref[:, :3]).max() > 0
+
+
+def test_circulation_over_steps_against_oracle(native, oracle):
+    """Three RK2 steps with the baroclinic source (smoke = the vortices),
+    run independently on the GPU (mixed precision) and by the oracle (FP64):
+    total circulation sum(w) agrees to 1e-6 of its magnitude, although the
+    two trajectories are not compared step by step."""
+    DB = native.DeviceBuffer
+    n, g, steps = 12_000, 32, 3
+    pos, w = scenes.ring(n, seed=39)
+    cent = np.array(pos, np.float32)
+    w = np.array(w, np.float64) * 1e3
+    rng = np.random.default_rng(40)
+    smoke = rng.random(n).astype(np.float32)
+    o, L = oracle.reference_domain(cent, g)
+    h = L / g
+    dt = 0.5 * h / 1e3
+    gvec = (0.0, 0.0, -1.0)
+    p = native.config_params(g, h=h, origin=o, precision=1, mg_tol=1e-9)
+    ctx = native.Context(p)
+    dens = native.Density()
+    d_c = DB.from_array(cent)
+    d_s = DB.from_array(smoke)
+    d_w = [DB.from_array(np.ascontiguousarray(w[c])) for c in range(3)]
+    c_ref, w_ref = cent, w
+    for s in range(steps):
+        print(f"[timestep] circulation run, step {s}", flush=True)
+        # on the context's stream, so the step reads the updated strengths
+        dens.add_baroclinic(d_c.ptr, d_s.ptr, n, d_c.ptr, [q.ptr for q in d_w], n, 5 * dt, gvec, 0.02, 200,
+                            ctx.stream)
+        ctx.time_step(n, d_c.ptr, [q.ptr for q in d_w], dt, 0.1, 0.1, keep_domain=True, rk2=True)
+        native.synchronize()
+        w_ref, _ = oracle.baroclinic(c_ref, smoke, c_ref, w_ref, 5 * dt, gvec, 0.02, 200)
+        c_ref, w_ref, _ = oracle.time_step(c_ref, w_ref, dt, 0.1, 0.1, g, K=p.K, kernel=p.kernel, core_a=p.core_a,
+                                           tol=1e-12, domain=(o, h), rk2=True)
+    w_gpu = np.stack([q.to_array(np.float64) for q in d_w])
+    circ0, circ_gpu, circ_ref = w.sum(1), w_gpu.sum(1), w_ref.sum(1)
+    scale = np.abs(w_ref).sum()
+    diff = np.abs(circ_gpu - circ_ref).max() / scale
+    print(f"  circulation: start {circ0}, oracle {circ_ref}, GPU {circ_gpu}; |diff| / sum|w| = {diff:.3g}",
+          flush=True)
+    assert diff < 1e-6
