@@ -80,6 +80,8 @@ WORKLOADS = {
     "uniform-64k-32": (uniform, 65536, 32),
     "ring-2M-128": (ring, 2097152, 128),
     "plume-8M-256": (plume, 8388608, 256),
+    # the long buoyant-flow run's size (plume as above)
+    "plume-32M-512": (plume, 33554432, 512),
 }
 
 
