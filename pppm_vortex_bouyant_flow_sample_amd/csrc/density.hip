@@ -300,8 +300,8 @@ extern "C" pppm_status pppm_add_baroclinic(pppm_density* d, const float* heat_po
     d->origin[0] = (mnx + 0.5 * (mxx - mnx)) - 0.5 * L;
     d->origin[1] = (mny + 0.5 * (mxy - mny)) - 0.5 * L;
     d->origin[2] = (mnz + 0.5 * (mxz - mnz)) - 0.5 * L;
-    uint32_t dim = (uint32_t)std::floor(L / cell_target) + 1;
-    dim = std::min(dim, max_dim);
+    // floor(L / cell) + 1, capped at max_dim before the conversion
+    const uint32_t dim = (uint32_t)std::min(std::floor(L / cell_target) + 1.0, (double)max_dim);
     const double h = L / (double)dim;
     d->dim = dim;
     d->h = h;

@@ -387,3 +387,38 @@ def test_circulation_over_steps_against_oracle(native, oracle):
     print(f"  circulation: start {circ0}, oracle {circ_ref}, GPU {circ_gpu}; |diff| / sum|w| = {diff:.3g}",
           flush=True)
     assert diff < 1e-6
+
+
+def test_time_step_edge_cases(native):
+    """One vortex, zero strengths (zero-length segments) and the error paths."""
+    DB = native.DeviceBuffer
+    p = native.config_params(16, precision=1)
+    ctx = native.Context(p)
+    # zero strengths: the ends coincide with the centre, nothing moves
+    n = 500
+    rng = np.random.default_rng(41)
+    cent = np.zeros((n, 4), np.float32)
+    cent[:, :3] = rng.random((n, 3)).astype(np.float32)
+    d_c = DB.from_array(cent)
+    d_w = [DB.from_array(np.zeros(n)) for _ in range(3)]
+    print("[timestep] edge cases", flush=True)
+    ctx.time_step(n, d_c.ptr, [q.ptr for q in d_w], 1e-3, 0.1, 0.1)
+    native.synchronize()
+    np.testing.assert_array_equal(d_c.to_array(np.float32).reshape(n, 4)[:, :3], cent[:, :3])
+    assert not np.any(np.stack([q.to_array(np.float64) for q in d_w]))
+    # a single vortex runs (the domain rule needs an extent, so the fixed
+    # grid); it moves only by the mesh's residual self-field, at most dt times
+    # a velocity of order one here
+    one = np.array([[0.5, 0.5, 0.5, 1.0]], np.float32)
+    d_1 = DB.from_array(one)
+    d_w1 = [DB.from_array(np.array([v])) for v in (0.3, -0.2, 0.05)]
+    ctx.time_step(1, d_1.ptr, [q.ptr for q in d_w1], 1e-3, 0.1, 0.1, keep_domain=True, rk2=True)
+    native.synchronize()
+    moved = d_1.to_array(np.float32)[:3]
+    assert np.isfinite(moved).all() and np.abs(moved - one[0, :3]).max() < 1e-2
+    with pytest.raises(native.PPPMError):
+        ctx.time_step(n, d_c.ptr, [q.ptr for q in d_w], 1e-3, 0.0, 0.1)
+    with pytest.raises(native.PPPMError):
+        native.check(native.lib().pppm_time_step(ctx.h, native.C.c_uint32(n), d_c.ptr, d_w[0].ptr, d_w[1].ptr,
+                                                 d_w[2].ptr, native.C.c_double(1e-3), native.C.c_double(0.1),
+                                                 native.C.c_double(0.1), native.C.c_int32(8)), "time_step")
