@@ -92,6 +92,36 @@ void GradDensity(float *phi, float3 *dphi, double cell_h, uint32_t dimx, uint32_
 void ParticleGetBaroclinic(float4 *vortex_pos, double *vort_x, double *vort_y, double *vort_z, float3 *grad_rho,
                            double dt_beta, uint32_t num_vort, uint32_t dimx, uint32_t dimy, uint32_t dimz,
                            double inv_h, double4 origin);
+/* Potential-flow gradient path (PotentialFieldSolver.cpp:441-739
+ * computeFarFieldBuffer, evaluateGradient), ParticleMesh.h:43-120, 245;
+ * kernels GPU_kernel.cu:560-610, 750-912, 964-987, 988-1200, 1520-1594,
+ * 1678-1718, 2977-3117.  P2M, RHS and the FMG solves are ParticleToMesh,
+ * ComputeRHS and MultiGridSolver3DPeriod_double above.
+ *  ComputeGradient: dphi = direction * periodic central differences of phi.
+ *  applyDirichlet: RHS -= M / (4 pi r) at each boundary face's ghost centre.
+ *  ComputeDiagPhi: phi = 0.250580605913965 gridval h^2 G.
+ *  PotentialComputeFarField: pforce = gforce - sum_{D != 0} dir[D] G direction
+ *    rho h^3 - grad(diagonal phi); phi and gforce are overwritten.
+ *  PotentialInterpolateFarField: hat-weighted far field at in-grid points.
+ *  Potential_PPCorrMN: pforce += G direction d m / (4 pi max(r, 0.05)^3) over
+ *    masses in the box |d|inf < (K + 1/2) h.
+ *  PotentialComputeGradForOutParticle: zero gradient outside [bbmin, bbmax]
+ *    (the reference computes the monopole there and stores zero). */
+void ComputeGradient(double *phi, double3 *dphi, double cell_h, double direction, uint32_t dimx, uint32_t dimy,
+                     uint32_t dimz);
+void applyDirichlet(double *RHS, double4 center, double monopole, double4 grid_coner, double h, uint32_t dimx,
+                    uint32_t dimy, uint32_t dimz);
+void ComputeDiagPhi(double *gridval, double *phi, double h_sqr, double G, uint32_t num_cell);
+void PotentialComputeFarField(uint32_t *start, uint32_t *end, float4 *pos, double *mass, double *gridval, double *phi,
+                              double3 *gforce, double3 *pforce, double cell_h, double G, double direction,
+                              uint3 gdim, uint3 hdim, int K, uint32_t num_particles, double4 origin);
+void PotentialInterpolateFarField(float4 *pos, double3 *far_force, double3 *p_force, double cell_h, uint32_t dimx,
+                                  uint32_t dimy, uint32_t dimz, uint32_t num_particles, double4 origin);
+void Potential_PPCorrMN(uint32_t *start, uint32_t *end, float4 *pos_eval, float4 *pos_mass, double *mass,
+                        double3 *pforce, double invh, double h, double G, double direction, uint3 gdim, uint3 hdim,
+                        uint2 gstride, uint2 hstride, int K, uint32_t M, uint32_t N, double4 origin);
+void PotentialComputeGradForOutParticle(float4 *pos, double total_mass, double3 mass_center, float3 bbmin,
+                                        float3 bbmax, double G, double direction, double3 *p_force, uint32_t M);
 }
 
 #endif

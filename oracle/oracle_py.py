@@ -400,3 +400,59 @@ def baroclinic(heat_pos4, heat_rho, cent4, omega, dt_beta, g=(0.0, 10.0, 0.0), c
                          C.c_uint32(cent4.shape[0]), C.c_uint32(dim), C.c_double(1.0 / float(np.float32(h))), o, gv)
     return np.stack(w), {"dim": dim, "h": h, "origin": origin, "rho": grid, "grad": grad.reshape(-1, 3),
                          "start": start, "end": end, "spos": spos, "srho": srho}
+
+
+# ---- potential-flow gradient path -------------------------------------------------
+def potential_gradient(mass_pos4, mass, eval_pos4, g, K=2, G=1.0, direction=1.0, fmg_passes=3):
+    """PotentialFieldSolver computeFarFieldBuffer + evaluateGradient
+    (PotentialFieldSolver.cpp:441-739): domain 2 x the masses' extent, mass
+    hash, P2M, RHS -h^2 rho (h^2 a float product), the monopole Dirichlet
+    values, three reference FMG passes, grad phi, the far-field buffer,
+    interpolation, near field and the outside-box rule.  Returns
+    (grad (M, 3), stages dict)."""
+    mass_pos4 = _f32(mass_pos4)
+    eval_pos4 = _f32(eval_pos4)
+    mass = _f64(mass)
+    n, m = mass_pos4.shape[0], eval_pos4.shape[0]
+    p3 = mass_pos4[:, :3].astype(np.float64)
+    mn, mx = p3.min(0), p3.max(0)
+    L = 2.0 * (mx - mn).max()
+    origin = [float(mn[c] + 0.5 * (mx[c] - mn[c]) - 0.5 * L) for c in range(3)]
+    hf = float(np.float32(L / g))
+    of = [float(np.float32(v)) for v in origin]
+    key, idx = hash_keys(mass_pos4, g, hf, of)
+    key, idx = sort_pairs(key, idx)
+    start, end = cell_tables(key, g ** 3)
+    spos = np.ascontiguousarray(mass_pos4[idx])
+    smass = np.ascontiguousarray(mass[idx])
+    rho = p2m(start, end, spos, smass, hf, g, origin)
+    h2 = float(np.float32(hf) * np.float32(hf))
+    rhs = rho * h2 * -1.0
+    tw = np.abs(smass).sum() if n else 0.0
+    total = smass.sum()
+    cen = [float((spos[:, c].astype(np.float64) * np.abs(smass)).sum() / tw) for c in range(3)]
+    rhs = np.ascontiguousarray(rhs)
+    cc = (C.c_double * 3)(*cen)
+    oc = (C.c_double * 3)(*origin)
+    lib().orc_pot_dirichlet(_p(rhs), cc, C.c_double(total), oc, C.c_double(hf), C.c_int(g))
+    dims = (g, g, g)
+    phi = None
+    for _ in range(fmg_passes):
+        phi = mg_solve(rhs, dims, schedule=MG_REF_FMG, nu1=4, nu2=4, omega=float(np.float32(1.43)), x0=phi)[0]
+    gforce = np.empty(3 * g ** 3)
+    lib().orc_pot_gradient(_p(phi), _p(gforce), C.c_double(hf), C.c_double(1.0), C.c_int(g))
+    grad_grid = gforce.copy()
+    phi2 = np.empty(g ** 3)
+    far = np.empty(3 * g ** 3)
+    lib().orc_pot_far_field(_p(rho), _p(phi2), _p(gforce), _p(far), C.c_double(hf), C.c_double(G),
+                            C.c_double(direction), C.c_int(g), C.c_int(K))
+    pf = np.zeros(3 * m)
+    lib().orc_pot_interp(_p(eval_pos4), _p(far), _p(pf), C.c_double(hf), C.c_int(g), C.c_uint32(m), oc)
+    lib().orc_pot_near(_p(start), _p(end), _p(eval_pos4), _p(spos), _p(smass), _p(pf), C.c_double(hf),
+                       C.c_double(direction * G), C.c_int(g), C.c_int(K), C.c_uint32(m), C.c_uint32(n), oc)
+    lo = (C.c_float * 3)(*of)
+    hi = (C.c_float * 3)(*[float(np.float32(of[c] + L)) for c in range(3)])
+    lib().orc_pot_outside(_p(eval_pos4), lo, hi, _p(pf), C.c_uint32(m))
+    return pf.reshape(m, 3), {"origin": origin, "L": L, "h": hf, "rho": rho, "rhs": rhs, "phi": phi,
+                              "grad_grid": grad_grid.reshape(-1, 3), "far": far.reshape(-1, 3),
+                              "center": cen, "total": total}

@@ -1420,3 +1420,181 @@ void orc_baroclinic(const float *vpos4, double *wx, double *wy, double *wz, cons
         wz[t] += dt_beta * 0.1 * bz;
     }
 }
+
+/* ---- potential-flow gradient path (PotentialFieldSolver) ----------------
+ * PotentialFieldSolver.cpp:441-739 and GPU_kernel.cu:560-610, 750-912,
+ * 964-1200, 1520-1594, 1678-1718, 2977-3117.  Vectors are (x, y, z) triples. */
+
+/* compute_gradient_kernel (GPU_kernel.cu:560-587): periodic central differences. */
+void orc_pot_gradient(const double *phi, double *dphi, double h, double direction, int n)
+{
+    const double invh = 1.0 / h;
+    int64_t c;
+    for (c = 0; c < (int64_t)n * n * n; c++) {
+        int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((int64_t)n * n));
+        int im = (i + n - 1) % n, ip = (i + 1) % n, jm = (j + n - 1) % n, jp = (j + 1) % n;
+        int km = (k + n - 1) % n, kp = (k + 1) % n;
+#define PH(a, b, e) phi[((int64_t)(e) * n + (b)) * n + (a)]
+        dphi[3 * c] = direction * (PH(ip, j, k) - PH(im, j, k)) * 0.5 * invh;
+        dphi[3 * c + 1] = direction * (PH(i, jp, k) - PH(i, jm, k)) * 0.5 * invh;
+        dphi[3 * c + 2] = direction * (PH(i, j, kp) - PH(i, j, km)) * 0.5 * invh;
+#undef PH
+    }
+}
+
+static double orc_four_pi_r(double x, double y, double z, const double c[3])
+{
+    double r2 = (x - c[0]) * (x - c[0]) + (y - c[1]) * (y - c[1]) + (z - c[2]) * (z - c[2]);
+    return r2 > 1e-12 ? 12.5663706144 * sqrt(r2) : 1e+12;
+}
+
+/* apply_Dirichlet_kernel (GPU_kernel.cu:2977-3101): rhs -= M / (4 pi r) at the
+ * ghost centre of each boundary face, faces in the order x-, x+, y-, y+, z-, z+. */
+void orc_pot_dirichlet(double *rhs, const double center[3], double monopole, const double corner[3], double h, int n)
+{
+    int64_t c;
+    for (c = 0; c < (int64_t)n * n * n; c++) {
+        int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((int64_t)n * n));
+        double g[6][3] = {{i - 1.0 + 0.5, j + 0.5, k + 0.5}, {i + 1.0 + 0.5, j + 0.5, k + 0.5},
+                          {i + 0.5, j - 1.0 + 0.5, k + 0.5}, {i + 0.5, j + 1.0 + 0.5, k + 0.5},
+                          {i + 0.5, j + 0.5, k - 1.0 + 0.5}, {i + 0.5, j + 0.5, k + 1.0 + 0.5}};
+        int on[6] = {i == 0, i == n - 1, j == 0, j == n - 1, k == 0, k == n - 1};
+        int f;
+        for (f = 0; f < 6; f++)
+            if (on[f]) {
+                double x = g[f][0] * h + corner[0], y = g[f][1] * h + corner[1], z = g[f][2] * h + corner[2];
+                rhs[c] -= monopole / orc_four_pi_r(x, y, z, center);
+            }
+    }
+}
+
+/* PotentialComputeFarField (GPU_kernel.cu:1098-1200): far = gforce - sum_{D != 0}
+ * dir[D] (G direction) rho[c+D] h^3; then phi = 0.250580605913965 rho h^2 G,
+ * gforce = its gradient (direction), far -= gforce. */
+void orc_pot_far_field(const double *rho, double *phi, double *gforce, double *far, double h, double G,
+                       double direction, int n, int K)
+{
+    int W = 2 * K + 1;
+    double *dir = (double *)malloc(sizeof(double) * 3 * W * W * W);
+    int64_t c, G3 = (int64_t)n * n * n;
+    orc_far_dir_table(K, h, dir);
+#pragma omp parallel for schedule(static)
+    for (c = 0; c < G3; c++) {
+        int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((int64_t)n * n));
+        double sx = 0, sy = 0, sz = 0;
+        int ii, jj, kk;
+        for (kk = k - K; kk <= k + K; kk++)
+            for (jj = j - K; jj <= j + K; jj++)
+                for (ii = i - K; ii <= i + K; ii++) {
+                    const double *d;
+                    double m;
+                    if (!(kk >= 0 && kk < n && jj >= 0 && jj < n && ii >= 0 && ii < n)) continue;
+                    if (kk == k && jj == j && ii == i) continue;
+                    m = (G * direction) * rho[((int64_t)kk * n + jj) * n + ii] * (h * h * h);
+                    d = dir + 3 * (((kk - k + K) * W + (jj - j + K)) * W + (ii - i + K));
+                    sx += d[0] * m;
+                    sy += d[1] * m;
+                    sz += d[2] * m;
+                }
+        far[3 * c] = gforce[3 * c] - sx;
+        far[3 * c + 1] = gforce[3 * c + 1] - sy;
+        far[3 * c + 2] = gforce[3 * c + 2] - sz;
+    }
+    for (c = 0; c < G3; c++) phi[c] = 0.250580605913965 * rho[c] * (h * h) * G;
+    orc_pot_gradient(phi, gforce, h, direction, n);
+    for (c = 0; c < 3 * G3; c++) far[c] -= gforce[c];
+    free(dir);
+}
+
+/* PotentialInterpolate_far_field_kernel2 (GPU_kernel.cu:1520-1576) */
+void orc_pot_interp(const float *pos4, const double *far, double *pf, double h, int n, uint32_t m,
+                    const double origin[3])
+{
+    const double invh = 1.0 / h;
+    int64_t t;
+#pragma omp parallel for schedule(static)
+    for (t = 0; t < (int64_t)m; t++) {
+        float px = (float)((double)pos4[4 * t] - origin[0]);
+        float py = (float)((double)pos4[4 * t + 1] - origin[1]);
+        float pz = (float)((double)pos4[4 * t + 2] - origin[2]);
+        double fi = floor(px * invh), fj = floor(py * invh), fk = floor(pz * invh);
+        double qx = px * invh, qy = py * invh, qz = pz * invh, sx = 0, sy = 0, sz = 0;
+        int i, j, k, ii, jj, kk;
+        if (!(fi >= 0 && fi < n && fj >= 0 && fj < n && fk >= 0 && fk < n)) continue;
+        i = (int)fi;
+        j = (int)fj;
+        k = (int)fk;
+        for (kk = k - 2; kk <= k + 2; kk++)
+            for (jj = j - 2; jj <= j + 2; jj++)
+                for (ii = i - 2; ii <= i + 2; ii++) {
+                    int64_t q;
+                    double w;
+                    if (!(kk >= 0 && kk < n && jj >= 0 && jj < n && ii >= 0 && ii < n)) continue;
+                    q = ((int64_t)kk * n + jj) * n + ii;
+                    w = orc_poly_shape((double)(float)(ii + 0.5) - qx) * orc_poly_shape((double)(float)(jj + 0.5) - qy) *
+                        orc_poly_shape((double)(float)(kk + 0.5) - qz);
+                    sx += far[3 * q] * w;
+                    sy += far[3 * q + 1] * w;
+                    sz += far[3 * q + 2] * w;
+                }
+        pf[3 * t] = sx;
+        pf[3 * t + 1] = sy;
+        pf[3 * t + 2] = sz;
+    }
+}
+
+/* Potential_PPCorrMN_kernel (GPU_kernel.cu:750-870), G already times direction */
+void orc_pot_near(const uint32_t *start, const uint32_t *end, const float *pe4, const float *pm4, const double *mass,
+                  double *pf, double h, double G, int n, int K, uint32_t M, uint32_t N, const double origin[3])
+{
+    const double invh = 1.0 / h, R = ((double)K + 0.5) / invh, a = 0.05;
+    int64_t t;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (t = 0; t < (int64_t)M; t++) {
+        float ax = (float)((double)pe4[4 * t] - origin[0]);
+        float ay = (float)((double)pe4[4 * t + 1] - origin[1]);
+        float az = (float)((double)pe4[4 * t + 2] - origin[2]);
+        int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+        double sx = 0, sy = 0, sz = 0;
+        int ii, jj, kk;
+        for (kk = k - K - 1; kk <= k + K + 1; kk++)
+            for (jj = j - K - 1; jj <= j + K + 1; jj++)
+                for (ii = i - K - 1; ii <= i + K + 1; ii++) {
+                    uint32_t q, p;
+                    if (!(kk >= 0 && kk < n && jj >= 0 && jj < n && ii >= 0 && ii < n)) continue;
+                    q = (uint32_t)(((int64_t)kk * n + jj) * n + ii);
+                    for (p = start[q]; p < end[q]; p++) {
+                        float bx, by, bz;
+                        double dx, dy, dz, r1, r3;
+                        if (p >= N) continue;
+                        bx = (float)((double)pm4[4 * (size_t)p] - origin[0]);
+                        by = (float)((double)pm4[4 * (size_t)p + 1] - origin[1]);
+                        bz = (float)((double)pm4[4 * (size_t)p + 2] - origin[2]);
+                        if (!((double)fabsf(bx - ax) < R && (double)fabsf(by - ay) < R && (double)fabsf(bz - az) < R))
+                            continue;
+                        dx = (double)(float)(bx - ax);
+                        dy = (double)(float)(by - ay);
+                        dz = (double)(float)(bz - az);
+                        r1 = sqrt(dx * dx + dy * dy + dz * dz);
+                        r3 = r1 > a ? 12.5663706144 * r1 * r1 * r1 : (r1 < 1e-12 ? 1e+14 : 12.5663706144 * a * a * a);
+                        sx += G * dx / r3 * mass[p];
+                        sy += G * dy / r3 * mass[p];
+                        sz += G * dz / r3 * mass[p];
+                    }
+                }
+        pf[3 * t] = pf[3 * t] + sx;
+        pf[3 * t + 1] = pf[3 * t + 1] + sy;
+        pf[3 * t + 2] = pf[3 * t + 2] + sz;
+    }
+}
+
+/* Potentialparticle_outside_kernel (GPU_kernel.cu:1678-1709): zero outside the box */
+void orc_pot_outside(const float *pos4, const float lo[3], const float hi[3], double *pf, uint32_t M)
+{
+    uint32_t t;
+    for (t = 0; t < M; t++) {
+        double x = pos4[4 * (size_t)t], y = pos4[4 * (size_t)t + 1], z = pos4[4 * (size_t)t + 2];
+        if (x < lo[0] || x > hi[0] || y < lo[1] || y > hi[1] || z < lo[2] || z > hi[2])
+            pf[3 * (size_t)t] = pf[3 * (size_t)t + 1] = pf[3 * (size_t)t + 2] = 0.0;
+    }
+}

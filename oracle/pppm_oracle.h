@@ -132,6 +132,17 @@ void orc_grad_density(const float *phi, uint32_t dim, double cell_h, float *grad
 void orc_baroclinic(const float *vpos4, double *wx, double *wy, double *wz, const float *grad, double dt_beta,
                     uint32_t n, uint32_t dim, double inv_h, const double origin[3], const double g[3]);
 
+/* ---- potential-flow gradient path (PotentialFieldSolver) ------------- */
+void orc_pot_gradient(const double *phi, double *dphi, double h, double direction, int n);
+void orc_pot_dirichlet(double *rhs, const double center[3], double monopole, const double corner[3], double h, int n);
+void orc_pot_far_field(const double *rho, double *phi, double *gforce, double *far, double h, double G,
+                       double direction, int n, int K);
+void orc_pot_interp(const float *pos4, const double *far, double *pf, double h, int n, uint32_t m,
+                    const double origin[3]);
+void orc_pot_near(const uint32_t *start, const uint32_t *end, const float *pe4, const float *pm4, const double *mass,
+                  double *pf, double h, double G, int n, int K, uint32_t M, uint32_t N, const double origin[3]);
+void orc_pot_outside(const float *pos4, const float lo[3], const float hi[3], double *pf, uint32_t M);
+
 int  orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -27,7 +27,7 @@ FLAGS = [
 
 SOURCES = [
     "scan.hip", "hash.hip", "p2m.hip", "mg.hip", "farfield.hip", "nearfield.hip",
-    "api.hip", "objects.hip", "ref_abi.hip", "serve.hip", "particle_mesh.hip", "frames.hip", "timestep.hip", "density.hip",
+    "api.hip", "objects.hip", "ref_abi.hip", "serve.hip", "particle_mesh.hip", "frames.hip", "timestep.hip", "density.hip", "potential.hip",
 ]
 
 
@@ -64,7 +64,7 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
-DROPIN_PROGRAMS = ["dropin_main", "pm_main", "ts_main", "bc_main"]
+DROPIN_PROGRAMS = ["dropin_main", "pm_main", "ts_main", "bc_main", "pot_main"]
 
 
 def build_dropin_test():

@@ -9,6 +9,11 @@
 #include "pppm_internal.hpp"
 #include "../../include/pppm_particle_mesh.h"
 
+// No fused multiply-adds: a box weight's argument q / h - (i + 1/2) rounds
+// its product first, as the reference's expression reads, so a particle on
+// a cell face deposits into the same cell as in the oracle.
+#pragma clang fp contract(off)
+
 namespace pppm {
 namespace {
 

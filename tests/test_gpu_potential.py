@@ -1,0 +1,67 @@
+"""The potential-flow gradient path (PotentialFieldSolver.cpp:409-739) through
+the reference's free functions, run by tests/dropin/pot_main.cpp in the
+reference's order, each stage against the oracle in FP64:
+
+  ParticleToMesh, ComputeRHS + applyDirichlet     rel 1e-12
+  three FMG passes, ComputeGradient               rel 1e-9
+  PotentialComputeFarField                        rel 1e-9
+  interpolation + Potential_PPCorrMN + outside    rel 1e-9 (outside points
+                                                  exactly zero)
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "dropin", "_build", "pot_main")
+
+
+@pytest.mark.parametrize("g,K", [(32, 2), (64, 1)])
+def test_potential_gradient_path(native, oracle, tmp_path, g, K):
+    assert os.path.exists(EXE), "build() first"
+    rng = np.random.default_rng(50 + g)
+    n, m = 40_000, 20_000
+    mp = np.zeros((n, 4), np.float32)
+    mp[:, :3] = (0.5 + 0.1 * rng.standard_normal((n, 3))).astype(np.float32)
+    mass = rng.standard_normal(n) / n + 1.0 / n
+    ev = np.zeros((m, 4), np.float32)
+    ev[:, :3] = (0.5 + 0.25 * rng.standard_normal((m, 3))).astype(np.float32)
+    with open(tmp_path / "in.bin", "wb") as f:
+        f.write(np.array([n, m, g], np.uint32).tobytes())
+        f.write(np.array([K], np.int32).tobytes())
+        for a in (mp, mass, ev):
+            f.write(np.ascontiguousarray(a).tobytes())
+    print(f"[potential] running {EXE} n={n} m={m} g={g} K={K}", flush=True)
+    r = subprocess.run([EXE, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(tmp_path / "out.bin", np.float64)
+    G = g ** 3
+    sizes = [G, G, G, 3 * G, 3 * G, 3 * m]
+    parts, off = [], 0
+    for s in sizes:
+        parts.append(raw[off:off + s])
+        off += s
+    assert off == raw.size
+    rho, rhs, phi, grad, far, pf = parts
+    pf_ref, st = oracle.potential_gradient(mp, mass, ev, g, K=K)
+    checks = [("ParticleToMesh", rho, st["rho"], 1e-12), ("ComputeRHS + applyDirichlet", rhs, st["rhs"], 1e-12),
+              ("three FMG passes", phi, st["phi"], 1e-9),
+              ("ComputeGradient", grad, st["grad_grid"].ravel(), 1e-9),
+              ("PotentialComputeFarField", far, st["far"].ravel(), 1e-9),
+              ("interpolation + Potential_PPCorrMN + outside", pf, pf_ref.ravel(), 1e-9)]
+    for name, got, ref, tol in checks:
+        e = rel_l2(got, ref)
+        print(f"  {name}: rel {e:.2e}", flush=True)
+        assert e < tol, name
+    # points outside the domain box get exactly zero, as in the reference
+    lo = np.float32(np.array(st["origin"], np.float32))
+    hi = (lo.astype(np.float64) + st["L"]).astype(np.float32)
+    out = ((ev[:, :3] < lo) | (ev[:, :3] > hi)).any(1)
+    assert out.any() and (~out).any()
+    assert not np.any(pf.reshape(m, 3)[out])
