@@ -122,6 +122,22 @@ void Potential_PPCorrMN(uint32_t *start, uint32_t *end, float4 *pos_eval, float4
                         uint2 gstride, uint2 hstride, int K, uint32_t M, uint32_t N, double4 origin);
 void PotentialComputeGradForOutParticle(float4 *pos, double total_mass, double3 mass_center, float3 bbmin,
                                         float3 bbmax, double G, double direction, double3 *p_force, uint32_t M);
+/* The scalar potential (PotentialFieldSolver.cpp:695-739,
+ * computeFarFieldPotentialBuffer and evaluateScalarPotential; kernels
+ * GPU_kernel.cu:1201-1300, 1301-1462, 1595-1676):
+ *  PotentialComputeScalarFarField: phi -= sum_{|D|inf <= K} w(D) G direction
+ *    rho h^2, w = 1/4 at D = 0, else 1 / (4 pi |D|) in cells.
+ *  PotentialInterpolateFarFieldScalar: hat-weighted phi at in-grid points.
+ *  Potential_PPCorrMNScalar: p_phi += G direction m / (4 pi max(r, 0.125)). */
+void PotentialComputeScalarFarField(double *mass, double *gridval, double *phi, double cell_h, double G,
+                                    double direction, uint3 gdim, uint3 hdim, int K);
+void PotentialInterpolateFarFieldScalar(float4 *pos, double *far_force, double *p_force, double cell_h,
+                                        uint32_t dimx, uint32_t dimy, uint32_t dimz, uint32_t num_particles,
+                                        double4 origin);
+void Potential_PPCorrMNScalar(uint32_t *start, uint32_t *end, float4 *pos_eval, float4 *pos_mass, double *mass,
+                              double *p_phi, double invh, double h, double G, double direction, uint3 gdim,
+                              uint3 hdim, uint2 gstride, uint2 hstride, int K, uint32_t M, uint32_t N,
+                              double4 origin);
 }
 
 #endif

@@ -453,6 +453,14 @@ def potential_gradient(mass_pos4, mass, eval_pos4, g, K=2, G=1.0, direction=1.0,
     lo = (C.c_float * 3)(*of)
     hi = (C.c_float * 3)(*[float(np.float32(of[c] + L)) for c in range(3)])
     lib().orc_pot_outside(_p(eval_pos4), lo, hi, _p(pf), C.c_uint32(m))
+    # computeFarFieldPotentialBuffer + evaluateScalarPotential on the same phi
+    phis = np.ascontiguousarray(phi.copy())
+    lib().orc_pot_scalar_far(_p(rho), _p(phis), C.c_double(hf), C.c_double(direction * G), C.c_int(g), C.c_int(K))
+    ps = np.zeros(m)
+    lib().orc_pot_interp_scalar(_p(eval_pos4), _p(phis), _p(ps), C.c_double(hf), C.c_int(g), C.c_uint32(m), oc)
+    lib().orc_pot_near_scalar(_p(start), _p(end), _p(eval_pos4), _p(spos), _p(smass), _p(ps), C.c_double(hf),
+                              C.c_double(direction * G), C.c_int(g), C.c_int(K), C.c_uint32(m), C.c_uint32(n), oc)
     return pf.reshape(m, 3), {"origin": origin, "L": L, "h": hf, "rho": rho, "rhs": rhs, "phi": phi,
+                              "phi_far": phis, "phi_points": ps,
                               "grad_grid": grad_grid.reshape(-1, 3), "far": far.reshape(-1, 3),
                               "center": cen, "total": total}

@@ -1598,3 +1598,106 @@ void orc_pot_outside(const float *pos4, const float lo[3], const float hi[3], do
             pf[3 * (size_t)t] = pf[3 * (size_t)t + 1] = pf[3 * (size_t)t + 2] = 0.0;
     }
 }
+
+/* PotentialComputeFarFieldScalarStep1_kernel (GPU_kernel.cu:1201-1267), G
+ * already times direction */
+void orc_pot_scalar_far(const double *rho, double *phi, double h, double G, int n, int K)
+{
+    int64_t c;
+#pragma omp parallel for schedule(static)
+    for (c = 0; c < (int64_t)n * n * n; c++) {
+        int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((int64_t)n * n));
+        double sum = 0.0;
+        int ii, jj, kk;
+        for (kk = k - K; kk <= k + K; kk++)
+            for (jj = j - K; jj <= j + K; jj++)
+                for (ii = i - K; ii <= i + K; ii++) {
+                    double m, w;
+                    if (!(kk >= 0 && kk < n && jj >= 0 && jj < n && ii >= 0 && ii < n)) continue;
+                    m = G * rho[((int64_t)kk * n + jj) * n + ii] * (h * h);
+                    if (ii == i && jj == j && kk == k) {
+                        w = 0.25;
+                    } else {
+                        double x = (double)ii - (double)i, y = (double)jj - (double)j, z = (double)kk - (double)k;
+                        w = 1.0 / (12.5663706144 * sqrt(x * x + y * y + z * z));
+                    }
+                    sum += w * m;
+                }
+        phi[c] = phi[c] - sum;
+    }
+}
+
+/* PotentialInterpolate_far_fieldScalar_kernel2 (GPU_kernel.cu:1595-1640) */
+void orc_pot_interp_scalar(const float *pos4, const double *far, double *pf, double h, int n, uint32_t m,
+                           const double origin[3])
+{
+    const double invh = 1.0 / h;
+    int64_t t;
+    for (t = 0; t < (int64_t)m; t++) {
+        float px = (float)((double)pos4[4 * t] - origin[0]);
+        float py = (float)((double)pos4[4 * t + 1] - origin[1]);
+        float pz = (float)((double)pos4[4 * t + 2] - origin[2]);
+        double fi = floor(px * invh), fj = floor(py * invh), fk = floor(pz * invh);
+        double qx = px * invh, qy = py * invh, qz = pz * invh, sum = 0;
+        int i, j, k, ii, jj, kk;
+        if (!(fi >= 0 && fi < n && fj >= 0 && fj < n && fk >= 0 && fk < n)) continue;
+        i = (int)fi;
+        j = (int)fj;
+        k = (int)fk;
+        for (kk = k - 2; kk <= k + 2; kk++)
+            for (jj = j - 2; jj <= j + 2; jj++)
+                for (ii = i - 2; ii <= i + 2; ii++) {
+                    if (!(kk >= 0 && kk < n && jj >= 0 && jj < n && ii >= 0 && ii < n)) continue;
+                    sum += far[((int64_t)kk * n + jj) * n + ii] *
+                           (orc_poly_shape((double)(float)(ii + 0.5) - qx) * orc_poly_shape((double)(float)(jj + 0.5) - qy) *
+                            orc_poly_shape((double)(float)(kk + 0.5) - qz));
+                }
+        pf[t] = sum;
+    }
+}
+
+/* Potential_PPCorrMNScalar_kernel (GPU_kernel.cu:1301-1419), G already times
+ * direction; the visited cells reach K + 1 on the side of the point's half */
+void orc_pot_near_scalar(const uint32_t *start, const uint32_t *end, const float *pe4, const float *pm4,
+                         const double *mass, double *pf, double h, double G, int n, int K, uint32_t M, uint32_t N,
+                         const double origin[3])
+{
+    const double invh = 1.0 / h, R = ((double)K + 0.5) / invh, a = 0.125;
+    int64_t t;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (t = 0; t < (int64_t)M; t++) {
+        float ax = (float)((double)pe4[4 * t] - origin[0]);
+        float ay = (float)((double)pe4[4 * t + 1] - origin[1]);
+        float az = (float)((double)pe4[4 * t + 2] - origin[2]);
+        int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+        double px = ax * invh, py = ay * invh, pz = az * invh, s = 0;
+        int i0 = px > i + 0.5 ? i - K : i - K - 1, i1 = px > i + 0.5 ? i + K + 1 : i + K;
+        int j0 = py > j + 0.5 ? j - K : j - K - 1, j1 = py > j + 0.5 ? j + K + 1 : j + K;
+        int k0 = pz > k + 0.5 ? k - K : k - K - 1, k1 = pz > k + 0.5 ? k + K + 1 : k + K;
+        int ii, jj, kk;
+        for (kk = k0; kk <= k1; kk++)
+            for (jj = j0; jj <= j1; jj++)
+                for (ii = i0; ii <= i1; ii++) {
+                    uint32_t q, p;
+                    if (!(kk >= 0 && kk < n && jj >= 0 && jj < n && ii >= 0 && ii < n)) continue;
+                    q = (uint32_t)(((int64_t)kk * n + jj) * n + ii);
+                    for (p = start[q]; p < end[q]; p++) {
+                        float bx, by, bz;
+                        double dx, dy, dz, r1, r2;
+                        if (p >= N) continue;
+                        bx = (float)((double)pm4[4 * (size_t)p] - origin[0]);
+                        by = (float)((double)pm4[4 * (size_t)p + 1] - origin[1]);
+                        bz = (float)((double)pm4[4 * (size_t)p + 2] - origin[2]);
+                        if (!((double)fabsf(bx - ax) < R && (double)fabsf(by - ay) < R && (double)fabsf(bz - az) < R))
+                            continue;
+                        dx = (double)(float)(bx - ax);
+                        dy = (double)(float)(by - ay);
+                        dz = (double)(float)(bz - az);
+                        r1 = sqrt(dx * dx + dy * dy + dz * dz);
+                        r2 = r1 > a ? 12.5663706144 * r1 : (r1 < 1e-12 ? 1e+14 : 12.5663706144 * a);
+                        s += G / r2 * mass[p];
+                    }
+                }
+        pf[t] = pf[t] + s;
+    }
+}

@@ -143,6 +143,13 @@ void orc_pot_near(const uint32_t *start, const uint32_t *end, const float *pe4, 
                   double *pf, double h, double G, int n, int K, uint32_t M, uint32_t N, const double origin[3]);
 void orc_pot_outside(const float *pos4, const float lo[3], const float hi[3], double *pf, uint32_t M);
 
+void orc_pot_scalar_far(const double *rho, double *phi, double h, double G, int n, int K);
+void orc_pot_interp_scalar(const float *pos4, const double *far, double *pf, double h, int n, uint32_t m,
+                           const double origin[3]);
+void orc_pot_near_scalar(const uint32_t *start, const uint32_t *end, const float *pe4, const float *pm4,
+                         const double *mass, double *pf, double h, double G, int n, int K, uint32_t M, uint32_t N,
+                         const double origin[3]);
+
 int  orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -199,6 +199,100 @@ __global__ void __launch_bounds__(256) k_pot_outside(const float4* __restrict__ 
     }
 }
 
+// PotentialComputeFarFieldScalarStep1_kernel (GPU_kernel.cu:1201-1267):
+// phi -= sum_{|D|inf <= K, in grid} w(D) (G rho[c+D] h^2), w = 1/4 at D = 0,
+// else 1 / (4 pi |D|) (compute_G_ij, in cells).
+__global__ void __launch_bounds__(256) k_pot_scalar_far(const double* __restrict__ rho, double* __restrict__ phi,
+                                                        double G, double h, int K, int nx, int ny, int nz) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)nx * ny * nz) return;
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    double sum = 0.0;
+    for (int kk = k - K; kk <= k + K; kk++)
+        for (int jj = j - K; jj <= j + K; jj++)
+            for (int ii = i - K; ii <= i + K; ii++) {
+                if (kk < 0 || kk >= nz || jj < 0 || jj >= ny || ii < 0 || ii >= nx) continue;
+                const double m = G * rho[((size_t)kk * ny + jj) * nx + ii] * (h * h);
+                double w;
+                if (ii == i && jj == j && kk == k) {
+                    w = 0.25;
+                } else {
+                    const double x = (double)ii - (double)i, y = (double)jj - (double)j, z = (double)kk - (double)k;
+                    w = 1.0 / (kRefFourPi * sqrt(x * x + y * y + z * z));
+                }
+                sum += w * m;
+            }
+    phi[c] = phi[c] - sum;
+}
+
+// PotentialInterpolate_far_fieldScalar_kernel2 (GPU_kernel.cu:1595-1640)
+__global__ void __launch_bounds__(256) k_pot_interp_scalar(const float4* __restrict__ pos,
+                                                           const double* __restrict__ far, double* __restrict__ pf,
+                                                           double invh, int nx, int ny, int nz, uint32_t n, double ox,
+                                                           double oy, double oz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 a = pos[t];
+    const float px = (float)((double)a.x - ox), py = (float)((double)a.y - oy), pz = (float)((double)a.z - oz);
+    const double fi = floor(px * invh), fj = floor(py * invh), fk = floor(pz * invh);
+    if (!(fi >= 0 && fi < nx && fj >= 0 && fj < ny && fk >= 0 && fk < nz)) return;
+    const int i = (int)fi, j = (int)fj, k = (int)fk;
+    const double qx = px * invh, qy = py * invh, qz = pz * invh;
+    double sum = 0;
+    for (int kk = k - 2; kk <= k + 2; kk++)
+        for (int jj = j - 2; jj <= j + 2; jj++)
+            for (int ii = i - 2; ii <= i + 2; ii++) {
+                if (kk < 0 || kk >= nz || jj < 0 || jj >= ny || ii < 0 || ii >= nx) continue;
+                const double w = hat1((double)(float)(ii + 0.5) - qx) * hat1((double)(float)(jj + 0.5) - qy) *
+                                 hat1((double)(float)(kk + 0.5) - qz);
+                sum += far[((size_t)kk * ny + jj) * nx + ii] * w;
+            }
+    pf[t] = sum;
+}
+
+// Potential_PPCorrMNScalar_kernel (GPU_kernel.cu:1301-1419): phi += G m /
+// (4 pi max(r, a)), a = 0.125 (1e14 in the denominator below r = 1e-12),
+// over masses in the box |d|inf < (K + 1/2) h; the cells visited reach K + 1
+// only on the side of the point's half of its cell.
+__global__ void __launch_bounds__(256) k_pot_near_scalar(const uint32_t* __restrict__ start,
+                                                         const uint32_t* __restrict__ end,
+                                                         const float4* __restrict__ pe,
+                                                         const float4* __restrict__ pm,
+                                                         const double* __restrict__ mass, double* __restrict__ pf,
+                                                         double invh, double G, uint3 hd, int K, uint32_t M,
+                                                         uint32_t N, double ox, double oy, double oz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const double R = ((double)K + 0.5) / invh, a = 0.125;
+    const float4 A = pe[t];
+    const float ax = (float)((double)A.x - ox), ay = (float)((double)A.y - oy), az = (float)((double)A.z - oz);
+    const int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+    auto lo = [&](double q, int c) { return q > (double)c + 0.5 ? c - K : c - K - 1; };
+    auto hi = [&](double q, int c) { return q > (double)c + 0.5 ? c + K + 1 : c + K; };
+    const int i0 = lo(ax * invh, i), i1 = hi(ax * invh, i), j0 = lo(ay * invh, j), j1 = hi(ay * invh, j);
+    const int k0 = lo(az * invh, k), k1 = hi(az * invh, k);
+    double s = 0;
+    for (int kk = k0; kk <= k1; kk++)
+        for (int jj = j0; jj <= j1; jj++)
+            for (int ii = i0; ii <= i1; ii++) {
+                if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x) continue;
+                const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                for (uint32_t p = start[q]; p < end[q]; p++) {
+                    if (p >= N) continue;
+                    const float4 B = pm[p];
+                    const float bx = (float)((double)B.x - ox), by = (float)((double)B.y - oy),
+                                bz = (float)((double)B.z - oz);
+                    if (!(fabs((double)(bx - ax)) < R && fabs((double)(by - ay)) < R && fabs((double)(bz - az)) < R))
+                        continue;
+                    const double dx = (double)(bx - ax), dy = (double)(by - ay), dz = (double)(bz - az);
+                    const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
+                    const double r2 = r1 > a ? kRefFourPi * r1 : (r1 < 1e-12 ? 1e+14 : kRefFourPi * a);
+                    s += G / r2 * mass[p];
+                }
+            }
+    pf[t] = pf[t] + s;
+}
+
 // the far-field direction table (GPU_kernel.cu:1119-1150), the same central
 // difference of 1/(4 pi |X - X'|) as the vortex path's
 std::vector<double> pot_dir_table(int K, double h) {
@@ -314,4 +408,35 @@ extern "C" void PotentialComputeGradForOutParticle(float4* pos, double total_mas
     (void)direction;
     if (M == 0) return;
     hipLaunchKernelGGL(k_pot_outside, dim3(nblocks(M)), dim3(256), 0, 0, pos, bbmin, bbmax, (double*)p_force, M);
+}
+
+extern "C" void PotentialComputeScalarFarField(double* mass, double* gridval, double* phi, double cell_h, double G,
+                                               double direction, uint3 gdim, uint3 hdim, int K) {
+    (void)mass;
+    (void)hdim;
+    const size_t G3 = (size_t)gdim.x * gdim.y * gdim.z;
+    if (K < 0 || G3 == 0) return;
+    hipLaunchKernelGGL(k_pot_scalar_far, dim3(nblocks(G3)), dim3(256), 0, 0, gridval, phi, G * direction, cell_h, K,
+                       (int)gdim.x, (int)gdim.y, (int)gdim.z);
+}
+
+extern "C" void PotentialInterpolateFarFieldScalar(float4* pos, double* far_force, double* p_force, double cell_h,
+                                                   uint32_t dimx, uint32_t dimy, uint32_t dimz,
+                                                   uint32_t num_particles, double4 origin) {
+    if (num_particles == 0) return;
+    hipLaunchKernelGGL(k_pot_interp_scalar, dim3(nblocks(num_particles)), dim3(256), 0, 0, pos, far_force, p_force,
+                       1.0 / cell_h, (int)dimx, (int)dimy, (int)dimz, num_particles, origin.x, origin.y, origin.z);
+}
+
+extern "C" void Potential_PPCorrMNScalar(uint32_t* start, uint32_t* end, float4* pos_eval, float4* pos_mass,
+                                         double* mass, double* p_phi, double invh, double h, double G,
+                                         double direction, uint3 gdim, uint3 hdim, uint2 gstride, uint2 hstride,
+                                         int K, uint32_t M, uint32_t N, double4 origin) {
+    (void)invh;  // the reference recomputes 1/h
+    (void)gdim;
+    (void)gstride;
+    (void)hstride;
+    if (K < 0 || M == 0) return;
+    hipLaunchKernelGGL(k_pot_near_scalar, dim3(nblocks(M)), dim3(256), 0, 0, start, end, pos_eval, pos_mass, mass,
+                       p_phi, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y, origin.z);
 }

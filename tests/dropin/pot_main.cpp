@@ -6,13 +6,16 @@
 // extent), mass hash, ParticleToMesh, ComputeRHS, applyDirichlet with the
 // masses' total and centre, three FMG passes, ComputeGradient,
 // PotentialComputeFarField, PotentialInterpolateFarField, Potential_PPCorrMN,
-// PotentialComputeGradForOutParticle.  tests/test_gpu_potential.py checks
+// PotentialComputeGradForOutParticle; then the scalar potential
+// (PotentialComputeScalarFarField, PotentialInterpolateFarFieldScalar,
+// Potential_PPCorrMNScalar) on the same phi.  tests/test_gpu_potential.py checks
 // every stage with the oracle.
 //
 // usage: pot_main <in.bin> <out.bin>
 //   in.bin : uint32 n, uint32 m, uint32 g, int32 K, float4 mass_pos[n],
 //            double mass[n], float4 eval[m]
-//   out.bin: double rho[G], rhs[G], phi[G], grad[G][3], far[G][3], pf[m][3]
+//   out.bin: double rho[G], rhs[G], phi[G], grad[G][3], far[G][3], pf[m][3],
+//            phi_far[G], phi_points[m]
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -151,6 +154,17 @@ int main(int argc, char** argv) {
     ThreadSync();
     dump(far, 3 * G);
     dump(pf, 3 * (size_t)m);
+    // computeFarFieldPotentialBuffer + evaluateScalarPotential
+    // (PotentialFieldSolver.cpp:695-739) on the same phi
+    double* ps = dalloc<double>(m);
+    mg_zerofy(ps, 8 * (size_t)m);
+    PotentialComputeScalarFarField(d_smass, rho, phi, (double)hf, 1.0, 1.0, gd, gd, K);
+    PotentialInterpolateFarFieldScalar((float4*)d_ev, phi, ps, (double)hf, g, g, g, m, org);
+    Potential_PPCorrMNScalar(d_start, d_end, (float4*)d_ev, (float4*)d_smp, d_smass, ps, 1.0 / (double)hf,
+                             (double)hf, 1.0, 1.0, gd, gd, make_uint2(g * g, g), make_uint2(g * g, g), K, m, n, org);
+    ThreadSync();
+    dump(phi, G);
+    dump(ps, m);
     std::fclose(fo);
     const char* err = pppm_last_error();
     if (err && *err) std::fprintf(stderr, "last error: %s\n", err);

@@ -7,6 +7,7 @@ reference's order, each stage against the oracle in FP64:
   PotentialComputeFarField                        rel 1e-9
   interpolation + Potential_PPCorrMN + outside    rel 1e-9 (outside points
                                                   exactly zero)
+  the scalar potential on the same phi            rel 1e-9
 """
 import os
 import subprocess
@@ -42,19 +43,21 @@ def test_potential_gradient_path(native, oracle, tmp_path, g, K):
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(tmp_path / "out.bin", np.float64)
     G = g ** 3
-    sizes = [G, G, G, 3 * G, 3 * G, 3 * m]
+    sizes = [G, G, G, 3 * G, 3 * G, 3 * m, G, m]
     parts, off = [], 0
     for s in sizes:
         parts.append(raw[off:off + s])
         off += s
     assert off == raw.size
-    rho, rhs, phi, grad, far, pf = parts
+    rho, rhs, phi, grad, far, pf, phi_far, phi_pts = parts
     pf_ref, st = oracle.potential_gradient(mp, mass, ev, g, K=K)
     checks = [("ParticleToMesh", rho, st["rho"], 1e-12), ("ComputeRHS + applyDirichlet", rhs, st["rhs"], 1e-12),
               ("three FMG passes", phi, st["phi"], 1e-9),
               ("ComputeGradient", grad, st["grad_grid"].ravel(), 1e-9),
               ("PotentialComputeFarField", far, st["far"].ravel(), 1e-9),
-              ("interpolation + Potential_PPCorrMN + outside", pf, pf_ref.ravel(), 1e-9)]
+              ("interpolation + Potential_PPCorrMN + outside", pf, pf_ref.ravel(), 1e-9),
+              ("PotentialComputeScalarFarField", phi_far, st["phi_far"], 1e-9),
+              ("scalar interpolation + Potential_PPCorrMNScalar", phi_pts, st["phi_points"], 1e-9)]
     for name, got, ref, tol in checks:
         e = rel_l2(got, ref)
         print(f"  {name}: rel {e:.2e}", flush=True)

@@ -100,3 +100,34 @@ def test_potential_gradient_far_from_cloud(oracle):
     r = np.linalg.norm(d, axis=1)
     ref = -M * d / (4 * np.pi * r[:, None] ** 3)
     np.testing.assert_allclose(gr, ref, rtol=0.03, atol=0.03 * np.abs(ref).max())
+
+
+def test_pot_scalar_far_single_cell(oracle):
+    n, K, h, G = 9, 2, 0.1, 1.0
+    rho = np.zeros(n ** 3)
+    c0 = (4 * n + 4) * n + 4
+    rho[c0] = 2.0
+    phi = np.zeros(n ** 3)
+    oracle.lib().orc_pot_scalar_far(P(rho), P(phi), C.c_double(h), C.c_double(G), C.c_int(n), C.c_int(K))
+    phi = phi.reshape(n, n, n)
+    m = G * 2.0 * h * h
+    assert phi[4, 4, 4] == -0.25 * m
+    np.testing.assert_allclose(phi[4, 5, 6], -m / (12.5663706144 * np.sqrt(5.0)), rtol=1e-15)
+    assert phi[4, 4, 4 + K + 1] == 0.0
+
+
+def test_pot_near_scalar_two_masses(oracle):
+    g, K, h = 8, 1, 0.125
+    ev = np.zeros((2, 4), np.float32)
+    ev[:, :3] = [[0.52, 0.5, 0.5], [0.45, 0.5, 0.5]]
+    mp = np.zeros((1, 4), np.float32)
+    mp[0, :3] = [0.6, 0.5, 0.5]
+    key, idx = oracle.hash_keys(mp, g, h, [0.0, 0.0, 0.0])
+    st, en = oracle.cell_tables(key, g ** 3)
+    pf = np.zeros(2)
+    oracle.lib().orc_pot_near_scalar(P(st), P(en), P(ev), P(mp), P(np.array([3.0])), P(pf), C.c_double(h),
+                                     C.c_double(1.0), C.c_int(g), C.c_int(K), C.c_uint32(2), C.c_uint32(1),
+                                     (C.c_double * 3)(0, 0, 0))
+    # 0.08 away: inside the core a = 0.125, 3 / (4 pi a); 0.15 away: 3 / (4 pi r)
+    d1 = np.float64(np.float32(0.6) - np.float32(0.45))
+    np.testing.assert_allclose(pf, [3.0 / (12.5663706144 * 0.125), 3.0 / (12.5663706144 * d1)], rtol=1e-15)
