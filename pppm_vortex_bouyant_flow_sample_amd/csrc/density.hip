@@ -96,11 +96,11 @@ __global__ void __launch_bounds__(256) k_assign_density(const uint32_t* __restri
 // The same grid value with one workgroup per grid cell, for grids whose
 // cells hold many smoke particles (the reference's rule gives 7^3 cells for a
 // unit-sized cloud: a thread per cell would walk a seventh of the smoke
-// alone).  The threads stride over the sorted smoke of each of the 27
+// alone).  1024 threads stride over the sorted smoke of each of the 27
 // cells, and a fixed-order tree in LDS adds their partial sums, so the
 // result is deterministic and within one float rounding of the sequential
 // sum.
-constexpr int kDensityBlock = 256;
+constexpr int kDensityBlock = 1024;
 
 __global__ void __launch_bounds__(kDensityBlock) k_assign_density_block(
     const uint32_t* __restrict__ start, const uint32_t* __restrict__ end, const float4* __restrict__ pos,
