@@ -22,7 +22,16 @@ Three paths are timed on the same inputs:
 `value` is the served path's particle-velocities/s; the plain FP32 and
 sequential mixed figures and their p50/p99 latencies are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--slots S] [--no-cpu-baseline]
+After the paths, `--time-steps` (default 3) simulation steps are timed on the
+same vortices and reported beside the headline (not in `value`):
+`time_step` is RK2 (Heun) of the segment ends, `time_step_euler` the
+reference's forward Euler (at most 3 steps); each step is the baroclinic
+source (smoke = the vortices with the workload's smoke density) followed by
+pppm_time_step on the workload's fixed grid, with the change of total
+circulation over the timed steps.  `--workload plume-32M-512 --time-steps 100`
+is the long buoyant-flow run.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--slots S] [--time-steps T] [--no-cpu-baseline]
 """
 import argparse
 import collections
