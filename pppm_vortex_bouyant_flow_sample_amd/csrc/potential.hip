@@ -141,43 +141,90 @@ __global__ void __launch_bounds__(256) k_pot_interp(const float4* __restrict__ p
     pf[3 * (size_t)t + 2] = sz;
 }
 
+__device__ inline int pot_wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int pot_wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
 // Potential_PPCorrMN_kernel (GPU_kernel.cu:750-870): masses of the hash
 // cells within +-(K+1) inside the box |d|inf < (K + 1/2) h add
 // G d / (4 pi max(r, a)^3) m, a = 0.05; coincident masses (r < 1e-12) add
-// G d 1e-14 m (d = 0).
+// G d 1e-14 m (d = 0).  Points sorted by cell (the reference's large
+// evaluation hashes them) put a wave's lanes in a few neighbouring cells:
+// when the union of the lanes' cell ranges holds at most 2 (2K+3)^3 cells,
+// the wave walks it with uniform loads and each lane adds only the cells of
+// its own range -- the same cells in the same order, so the same sums.
 __global__ void __launch_bounds__(256) k_pot_near(const uint32_t* __restrict__ start,
                                                   const uint32_t* __restrict__ end, const float4* __restrict__ pe,
                                                   const float4* __restrict__ pm, const double* __restrict__ mass,
                                                   double* __restrict__ pf, double invh, double G, uint3 hd, int K,
                                                   uint32_t M, uint32_t N, double ox, double oy, double oz) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M) return;
+    const bool live = t < M;
     const double R = ((double)K + 0.5) / invh, a = 0.05;
-    const float4 A = pe[t];
+    const float4 A = pe[live ? t : M - 1];
     const float ax = (float)((double)A.x - ox), ay = (float)((double)A.y - oy), az = (float)((double)A.z - oz);
-    const int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+    // cells clamped to [-K-2, dim+K+1]: beyond, no in-grid cell is in range
+    auto cell = [&](float q, uint32_t d) {
+        return (int)fmin(fmax(floor(q * invh), -(double)K - 2.0), (double)d + (double)K + 1.0);
+    };
+    const int i = cell(ax, hd.x), j = cell(ay, hd.y), k = cell(az, hd.z);
     double sx = 0, sy = 0, sz = 0;
-    for (int kk = k - K - 1; kk <= k + K + 1; kk++)
-        for (int jj = j - K - 1; jj <= j + K + 1; jj++)
-            for (int ii = i - K - 1; ii <= i + K + 1; ii++) {
-                if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x) continue;
-                const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
-                for (uint32_t p = start[q]; p < end[q]; p++) {
-                    if (p >= N) continue;
-                    const float4 B = pm[p];
-                    const float bx = (float)((double)B.x - ox), by = (float)((double)B.y - oy),
-                                bz = (float)((double)B.z - oz);
-                    if (!(fabs((double)(bx - ax)) < R && fabs((double)(by - ay)) < R && fabs((double)(bz - az)) < R))
-                        continue;
-                    const double dx = (double)(bx - ax), dy = (double)(by - ay), dz = (double)(bz - az);
-                    const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
-                    const double r3 = r1 > a ? kRefFourPi * r1 * r1 * r1 : (r1 < 1e-12 ? 1e+14 : kRefFourPi * a * a * a);
-                    const double m = mass[p];
-                    sx += G * dx / r3 * m;
-                    sy += G * dy / r3 * m;
-                    sz += G * dz / r3 * m;
+    auto pair = [&](uint32_t p, bool use) {
+        const float4 B = pm[p];
+        const float bx = (float)((double)B.x - ox), by = (float)((double)B.y - oy), bz = (float)((double)B.z - oz);
+        if (!use || !(fabs((double)(bx - ax)) < R && fabs((double)(by - ay)) < R && fabs((double)(bz - az)) < R))
+            return;
+        const double dx = (double)(bx - ax), dy = (double)(by - ay), dz = (double)(bz - az);
+        const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
+        const double r3 = r1 > a ? kRefFourPi * r1 * r1 * r1 : (r1 < 1e-12 ? 1e+14 : kRefFourPi * a * a * a);
+        const double m = mass[p];
+        sx += G * dx / r3 * m;
+        sy += G * dy / r3 * m;
+        sz += G * dz / r3 * m;
+    };
+    const int E = K + 1;
+    const int X0 = __builtin_amdgcn_readfirstlane(max(pot_wave_min(i) - E, 0));
+    const int X1 = __builtin_amdgcn_readfirstlane(min(pot_wave_max(i) + E, (int)hd.x - 1));
+    const int Y0 = __builtin_amdgcn_readfirstlane(max(pot_wave_min(j) - E, 0));
+    const int Y1 = __builtin_amdgcn_readfirstlane(min(pot_wave_max(j) + E, (int)hd.y - 1));
+    const int Z0 = __builtin_amdgcn_readfirstlane(max(pot_wave_min(k) - E, 0));
+    const int Z1 = __builtin_amdgcn_readfirstlane(min(pot_wave_max(k) + E, (int)hd.z - 1));
+    const int64_t vol = (X1 >= X0 && Y1 >= Y0 && Z1 >= Z0) ? (int64_t)(X1 - X0 + 1) * (Y1 - Y0 + 1) * (Z1 - Z0 + 1)
+                                                           : 0;
+    const int64_t own = (int64_t)(2 * E + 1) * (2 * E + 1) * (2 * E + 1);
+    if (vol <= 2 * own) {
+        for (int kk = Z0; kk <= Z1; kk++)
+            for (int jj = Y0; jj <= Y1; jj++) {
+                const bool rin = abs(kk - k) <= E && abs(jj - j) <= E;
+                for (int ii = X0; ii <= X1; ii++) {
+                    const bool cin = rin && abs(ii - i) <= E;
+                    if (!__ballot(cin)) continue;
+                    const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                    const uint32_t p0 = __builtin_amdgcn_readfirstlane(start[q]);
+                    const uint32_t p1 = __builtin_amdgcn_readfirstlane(min(end[q], N));
+                    if (p0 == 0xFFFFFFFFu) continue;
+                    for (uint32_t p = p0; p < p1; p++) pair(p, cin);
                 }
             }
+    } else {
+        for (int kk = k - E; kk <= k + E; kk++)
+            for (int jj = j - E; jj <= j + E; jj++)
+                for (int ii = i - E; ii <= i + E; ii++) {
+                    if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x)
+                        continue;
+                    const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                    for (uint32_t p = start[q]; p < end[q]; p++) {
+                        if (p >= N) continue;
+                        pair(p, true);
+                    }
+                }
+    }
+    if (!live) return;
     pf[3 * (size_t)t] = pf[3 * (size_t)t] + sx;
     pf[3 * (size_t)t + 1] = pf[3 * (size_t)t + 1] + sy;
     pf[3 * (size_t)t + 2] = pf[3 * (size_t)t + 2] + sz;

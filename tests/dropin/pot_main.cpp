@@ -2,8 +2,8 @@
 // functions (include/pppm_particle_mesh.h, include/pppm_ref.h) in the order
 // of PotentialFieldSolver::computeFarFieldBuffer and evaluateGradient
 // (hybridFastSummation/hybridFastSummation/PotentialFieldSolver.cpp:409-739,
-// the evaluation without its large-batch hash): domain (2 x the masses'
-// extent), mass hash, ParticleToMesh, ComputeRHS, applyDirichlet with the
+// with the large evaluation, whose points are hashed and sorted): domain (2 x
+// the masses' extent), mass hash, ParticleToMesh, ComputeRHS, applyDirichlet with the
 // masses' total and centre, three FMG passes, ComputeGradient,
 // PotentialComputeFarField, PotentialInterpolateFarField, Potential_PPCorrMN,
 // PotentialComputeGradForOutParticle; then the scalar potential
@@ -142,11 +142,20 @@ int main(int argc, char** argv) {
     MemcpyDevToDev(phi2, phi, 8 * G);
     PotentialComputeFarField(d_start, d_end, (float4*)d_mp, d_smass, rho, phi2, (double3*)grad, (double3*)far,
                              (double)hf, 1.0, 1.0, gd, gd, K, m, org);
-    // evaluateGradient
-    mg_zerofy(pf, 24 * (size_t)m);
-    PotentialInterpolateFarField((float4*)d_ev, (double3*)far, (double3*)pf, (double)hf, g, g, g, m, org);
-    Potential_PPCorrMN(d_start, d_end, (float4*)d_ev, (float4*)d_smp, d_smass, (double3*)pf, 1.0 / (double)hf,
+    // evaluateGradient(true) (PotentialFieldSolver.cpp:441-485): the points
+    // hashed on the mass grid and sorted, interpolation and near field in
+    // that order, deordered into the caller's
+    uint32_t *d_ekey = dalloc<uint32_t>(m), *d_eidx = dalloc<uint32_t>(m);
+    float* d_sev = dalloc<float>(4 * (size_t)m);
+    double* pfs = dalloc<double>(3 * (size_t)m);
+    calcHash(d_ekey, d_eidx, d_ev, (int)m);
+    sortParticles(d_ekey, d_eidx, m);
+    reorder(d_eidx, d_ev, 4, d_sev, 1, m);
+    mg_zerofy(pfs, 24 * (size_t)m);
+    PotentialInterpolateFarField((float4*)d_sev, (double3*)far, (double3*)pfs, (double)hf, g, g, g, m, org);
+    Potential_PPCorrMN(d_start, d_end, (float4*)d_sev, (float4*)d_smp, d_smass, (double3*)pfs, 1.0 / (double)hf,
                        (double)hf, 1.0, 1.0, gd, gd, make_uint2(g * g, g), make_uint2(g * g, g), K, m, n, org);
+    deorder(d_eidx, pfs, 3, pf, 2, m);
     const float3 lo = make_float3((float)o[0], (float)o[1], (float)o[2]);
     const float3 hi = make_float3((float)((double)lo.x + L), (float)((double)lo.y + L), (float)((double)lo.z + L));
     PotentialComputeGradForOutParticle((float4*)d_ev, total, make_double3(cx, cy, cz), lo, hi, 1.0, 1.0,
