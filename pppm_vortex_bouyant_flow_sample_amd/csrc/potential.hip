@@ -300,43 +300,81 @@ __global__ void __launch_bounds__(256) k_pot_interp_scalar(const float4* __restr
 // Potential_PPCorrMNScalar_kernel (GPU_kernel.cu:1301-1419): phi += G m /
 // (4 pi max(r, a)), a = 0.125 (1e14 in the denominator below r = 1e-12),
 // over masses in the box |d|inf < (K + 1/2) h; the cells visited reach K + 1
-// only on the side of the point's half of its cell.
+// only on the side of the point's half of its cell.  Thread t takes point
+// order[t] (order null: t); with points in cell order, a wave walks the
+// union of its lanes' cell ranges with uniform loads when it holds at most
+// 2 (2K+3)^3 cells, each lane adding only its own cells in its own order.
 __global__ void __launch_bounds__(256) k_pot_near_scalar(const uint32_t* __restrict__ start,
                                                          const uint32_t* __restrict__ end,
                                                          const float4* __restrict__ pe,
                                                          const float4* __restrict__ pm,
                                                          const double* __restrict__ mass, double* __restrict__ pf,
-                                                         double invh, double G, uint3 hd, int K, uint32_t M,
-                                                         uint32_t N, double ox, double oy, double oz) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M) return;
+                                                         const uint32_t* __restrict__ order, double invh, double G,
+                                                         uint3 hd, int K, uint32_t M, uint32_t N, double ox,
+                                                         double oy, double oz) {
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t0 < M;
+    const uint32_t tc = live ? t0 : M - 1;
+    const uint32_t t = order ? order[tc] : tc;
     const double R = ((double)K + 0.5) / invh, a = 0.125;
     const float4 A = pe[t];
     const float ax = (float)((double)A.x - ox), ay = (float)((double)A.y - oy), az = (float)((double)A.z - oz);
-    const int i = (int)floor(ax * invh), j = (int)floor(ay * invh), k = (int)floor(az * invh);
+    auto cell = [&](float q, uint32_t d) {
+        return (int)fmin(fmax(floor(q * invh), -(double)K - 2.0), (double)d + (double)K + 1.0);
+    };
+    const int i = cell(ax, hd.x), j = cell(ay, hd.y), k = cell(az, hd.z);
     auto lo = [&](double q, int c) { return q > (double)c + 0.5 ? c - K : c - K - 1; };
     auto hi = [&](double q, int c) { return q > (double)c + 0.5 ? c + K + 1 : c + K; };
     const int i0 = lo(ax * invh, i), i1 = hi(ax * invh, i), j0 = lo(ay * invh, j), j1 = hi(ay * invh, j);
     const int k0 = lo(az * invh, k), k1 = hi(az * invh, k);
     double s = 0;
-    for (int kk = k0; kk <= k1; kk++)
-        for (int jj = j0; jj <= j1; jj++)
-            for (int ii = i0; ii <= i1; ii++) {
-                if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x) continue;
-                const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
-                for (uint32_t p = start[q]; p < end[q]; p++) {
-                    if (p >= N) continue;
-                    const float4 B = pm[p];
-                    const float bx = (float)((double)B.x - ox), by = (float)((double)B.y - oy),
-                                bz = (float)((double)B.z - oz);
-                    if (!(fabs((double)(bx - ax)) < R && fabs((double)(by - ay)) < R && fabs((double)(bz - az)) < R))
-                        continue;
-                    const double dx = (double)(bx - ax), dy = (double)(by - ay), dz = (double)(bz - az);
-                    const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
-                    const double r2 = r1 > a ? kRefFourPi * r1 : (r1 < 1e-12 ? 1e+14 : kRefFourPi * a);
-                    s += G / r2 * mass[p];
+    auto pair = [&](uint32_t p, bool use) {
+        const float4 B = pm[p];
+        const float bx = (float)((double)B.x - ox), by = (float)((double)B.y - oy), bz = (float)((double)B.z - oz);
+        if (!use || !(fabs((double)(bx - ax)) < R && fabs((double)(by - ay)) < R && fabs((double)(bz - az)) < R))
+            return;
+        const double dx = (double)(bx - ax), dy = (double)(by - ay), dz = (double)(bz - az);
+        const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
+        const double r2 = r1 > a ? kRefFourPi * r1 : (r1 < 1e-12 ? 1e+14 : kRefFourPi * a);
+        s += G / r2 * mass[p];
+    };
+    const int X0 = __builtin_amdgcn_readfirstlane(max(pot_wave_min(i0), 0));
+    const int X1 = __builtin_amdgcn_readfirstlane(min(pot_wave_max(i1), (int)hd.x - 1));
+    const int Y0 = __builtin_amdgcn_readfirstlane(max(pot_wave_min(j0), 0));
+    const int Y1 = __builtin_amdgcn_readfirstlane(min(pot_wave_max(j1), (int)hd.y - 1));
+    const int Z0 = __builtin_amdgcn_readfirstlane(max(pot_wave_min(k0), 0));
+    const int Z1 = __builtin_amdgcn_readfirstlane(min(pot_wave_max(k1), (int)hd.z - 1));
+    const int64_t vol = (X1 >= X0 && Y1 >= Y0 && Z1 >= Z0) ? (int64_t)(X1 - X0 + 1) * (Y1 - Y0 + 1) * (Z1 - Z0 + 1)
+                                                           : 0;
+    const int64_t own = (int64_t)(2 * K + 3) * (2 * K + 3) * (2 * K + 3);
+    if (vol <= 2 * own) {
+        for (int kk = Z0; kk <= Z1; kk++)
+            for (int jj = Y0; jj <= Y1; jj++) {
+                const bool rin = kk >= k0 && kk <= k1 && jj >= j0 && jj <= j1;
+                for (int ii = X0; ii <= X1; ii++) {
+                    const bool cin = rin && ii >= i0 && ii <= i1;
+                    if (!__ballot(cin)) continue;
+                    const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                    const uint32_t p0 = __builtin_amdgcn_readfirstlane(start[q]);
+                    const uint32_t p1 = __builtin_amdgcn_readfirstlane(min(end[q], N));
+                    if (p0 == 0xFFFFFFFFu) continue;
+                    for (uint32_t p = p0; p < p1; p++) pair(p, cin);
                 }
             }
+    } else {
+        for (int kk = k0; kk <= k1; kk++)
+            for (int jj = j0; jj <= j1; jj++)
+                for (int ii = i0; ii <= i1; ii++) {
+                    if (kk < 0 || kk >= (int)hd.z || jj < 0 || jj >= (int)hd.y || ii < 0 || ii >= (int)hd.x)
+                        continue;
+                    const uint32_t q = ((uint32_t)kk * hd.y + (uint32_t)jj) * hd.x + (uint32_t)ii;
+                    for (uint32_t p = start[q]; p < end[q]; p++) {
+                        if (p >= N) continue;
+                        pair(p, true);
+                    }
+                }
+    }
+    if (!live) return;
     pf[t] = pf[t] + s;
 }
 
@@ -484,6 +522,32 @@ extern "C" void Potential_PPCorrMNScalar(uint32_t* start, uint32_t* end, float4*
     (void)gstride;
     (void)hstride;
     if (K < 0 || M == 0) return;
+    // The points in cell order (keys clamped into the grid), so a wave's
+    // lanes share cells; each point's sum is the same in any order.  The
+    // temporaries live for the call, as the reference allocates per call.
+    uint32_t* order = nullptr;
+    uint32_t* buf = nullptr;
+    void* scr = nullptr;
+    const size_t cells = (size_t)hdim.x * hdim.y * hdim.z;
+    if (M >= 4096 && hdim.x == hdim.y && hdim.y == hdim.z && cells <= ((size_t)1 << 31) &&
+        hipMalloc((void**)&buf, 4 * sizeof(uint32_t) * (size_t)M) == hipSuccess &&
+        hipMalloc(&scr, sort_scratch_bytes(M)) == hipSuccess) {
+        HashGeom hg;
+        hg.g = hdim.x;
+        hg.hf = (float)h;
+        hg.of[0] = (float)origin.x;
+        hg.of[1] = (float)origin.y;
+        hg.of[2] = (float)origin.z;
+        uint32_t *ks = nullptr, *vs = nullptr;
+        if (launch_calc_keys(pos_eval, M, hg, buf, nullptr, 0, false, true) == PPPM_OK &&
+            radix_sort_pairs(buf, buf + M, buf + 2 * (size_t)M, buf + 3 * (size_t)M, M, key_bits_for(cells), true,
+                             scr, &ks, &vs, 0) == PPPM_OK)
+            order = vs;
+    } else {
+        (void)hipGetLastError();
+    }
     hipLaunchKernelGGL(k_pot_near_scalar, dim3(nblocks(M)), dim3(256), 0, 0, start, end, pos_eval, pos_mass, mass,
-                       p_phi, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y, origin.z);
+                       p_phi, order, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y, origin.z);
+    if (buf) (void)hipFree(buf);  // synchronises the device
+    if (scr) (void)hipFree(scr);
 }
