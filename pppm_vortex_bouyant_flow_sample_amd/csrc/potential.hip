@@ -154,19 +154,23 @@ __device__ inline int pot_wave_max(int v) {
 // cells within +-(K+1) inside the box |d|inf < (K + 1/2) h add
 // G d / (4 pi max(r, a)^3) m, a = 0.05; coincident masses (r < 1e-12) add
 // G d 1e-14 m (d = 0).  Points sorted by cell (the reference's large
-// evaluation hashes them) put a wave's lanes in a few neighbouring cells:
+// evaluation hashes them; thread t takes point order[t], order null: t) put
+// a wave's lanes in a few neighbouring cells:
 // when the union of the lanes' cell ranges holds at most 2 (2K+3)^3 cells,
 // the wave walks it with uniform loads and each lane adds only the cells of
 // its own range -- the same cells in the same order, so the same sums.
 __global__ void __launch_bounds__(256) k_pot_near(const uint32_t* __restrict__ start,
                                                   const uint32_t* __restrict__ end, const float4* __restrict__ pe,
                                                   const float4* __restrict__ pm, const double* __restrict__ mass,
-                                                  double* __restrict__ pf, double invh, double G, uint3 hd, int K,
-                                                  uint32_t M, uint32_t N, double ox, double oy, double oz) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = t < M;
+                                                  double* __restrict__ pf, const uint32_t* __restrict__ order,
+                                                  double invh, double G, uint3 hd, int K, uint32_t M, uint32_t N,
+                                                  double ox, double oy, double oz) {
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t0 < M;
+    const uint32_t tc = live ? t0 : M - 1;
+    const uint32_t t = order ? order[tc] : tc;
     const double R = ((double)K + 0.5) / invh, a = 0.05;
-    const float4 A = pe[live ? t : M - 1];
+    const float4 A = pe[t];
     const float ax = (float)((double)A.x - ox), ay = (float)((double)A.y - oy), az = (float)((double)A.z - oz);
     // cells clamped to [-K-2, dim+K+1]: beyond, no in-grid cell is in range
     auto cell = [&](float q, uint32_t d) {
@@ -378,6 +382,41 @@ __global__ void __launch_bounds__(256) k_pot_near_scalar(const uint32_t* __restr
     pf[t] = pf[t] + s;
 }
 
+// Cell order of M evaluation points on a cubic hash grid (keys clamped into
+// the grid), for the near fields: each point's sum is the same in any order,
+// and in cell order a wave's lanes share cells.  The buffers live until
+// release(); order stays null (identity) if they cannot be had.
+struct PointOrder {
+    uint32_t* order = nullptr;
+    uint32_t* buf = nullptr;
+    void* scr = nullptr;
+    PointOrder(const float4* pos, uint32_t M, uint3 hd, double h, double4 origin) {
+        const size_t cells = (size_t)hd.x * hd.y * hd.z;
+        if (M < 4096 || hd.x != hd.y || hd.y != hd.z || cells > ((size_t)1 << 31)) return;
+        if (hipMalloc((void**)&buf, 4 * sizeof(uint32_t) * (size_t)M) != hipSuccess ||
+            hipMalloc(&scr, sort_scratch_bytes(M)) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        HashGeom hg;
+        hg.g = hd.x;
+        hg.hf = (float)h;
+        hg.of[0] = (float)origin.x;
+        hg.of[1] = (float)origin.y;
+        hg.of[2] = (float)origin.z;
+        uint32_t *ks = nullptr, *vs = nullptr;
+        if (launch_calc_keys(pos, M, hg, buf, nullptr, 0, false, true) == PPPM_OK &&
+            radix_sort_pairs(buf, buf + M, buf + 2 * (size_t)M, buf + 3 * (size_t)M, M, key_bits_for(cells), true,
+                             scr, &ks, &vs, 0) == PPPM_OK)
+            order = vs;
+    }
+    // after the kernel that reads order: hipFree synchronises the device
+    ~PointOrder() {
+        if (buf) (void)hipFree(buf);
+        if (scr) (void)hipFree(scr);
+    }
+};
+
 // the far-field direction table (GPU_kernel.cu:1119-1150), the same central
 // difference of 1/(4 pi |X - X'|) as the vortex path's
 std::vector<double> pot_dir_table(int K, double h) {
@@ -480,8 +519,11 @@ extern "C" void Potential_PPCorrMN(uint32_t* start, uint32_t* end, float4* pos_e
     (void)gstride;
     (void)hstride;
     if (K < 0 || M == 0) return;
+    // the points in cell order for the call, whatever order the caller has
+    PointOrder po(pos_eval, M, hdim, h, origin);
     hipLaunchKernelGGL(k_pot_near, dim3(nblocks(M)), dim3(256), 0, 0, start, end, pos_eval, pos_mass, mass,
-                       (double*)pforce, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y, origin.z);
+                       (double*)pforce, po.order, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y,
+                       origin.z);
 }
 
 extern "C" void PotentialComputeGradForOutParticle(float4* pos, double total_mass, double3 mass_center, float3 bbmin,
@@ -522,32 +564,9 @@ extern "C" void Potential_PPCorrMNScalar(uint32_t* start, uint32_t* end, float4*
     (void)gstride;
     (void)hstride;
     if (K < 0 || M == 0) return;
-    // The points in cell order (keys clamped into the grid), so a wave's
-    // lanes share cells; each point's sum is the same in any order.  The
-    // temporaries live for the call, as the reference allocates per call.
-    uint32_t* order = nullptr;
-    uint32_t* buf = nullptr;
-    void* scr = nullptr;
-    const size_t cells = (size_t)hdim.x * hdim.y * hdim.z;
-    if (M >= 4096 && hdim.x == hdim.y && hdim.y == hdim.z && cells <= ((size_t)1 << 31) &&
-        hipMalloc((void**)&buf, 4 * sizeof(uint32_t) * (size_t)M) == hipSuccess &&
-        hipMalloc(&scr, sort_scratch_bytes(M)) == hipSuccess) {
-        HashGeom hg;
-        hg.g = hdim.x;
-        hg.hf = (float)h;
-        hg.of[0] = (float)origin.x;
-        hg.of[1] = (float)origin.y;
-        hg.of[2] = (float)origin.z;
-        uint32_t *ks = nullptr, *vs = nullptr;
-        if (launch_calc_keys(pos_eval, M, hg, buf, nullptr, 0, false, true) == PPPM_OK &&
-            radix_sort_pairs(buf, buf + M, buf + 2 * (size_t)M, buf + 3 * (size_t)M, M, key_bits_for(cells), true,
-                             scr, &ks, &vs, 0) == PPPM_OK)
-            order = vs;
-    } else {
-        (void)hipGetLastError();
-    }
+    // the points in cell order for the call (the reference allocates per
+    // call as well)
+    PointOrder po(pos_eval, M, hdim, h, origin);
     hipLaunchKernelGGL(k_pot_near_scalar, dim3(nblocks(M)), dim3(256), 0, 0, start, end, pos_eval, pos_mass, mass,
-                       p_phi, order, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y, origin.z);
-    if (buf) (void)hipFree(buf);  // synchronises the device
-    if (scr) (void)hipFree(scr);
+                       p_phi, po.order, 1.0 / h, direction * G, hdim, K, M, N, origin.x, origin.y, origin.z);
 }
